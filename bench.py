@@ -1,0 +1,170 @@
+"""Benchmark of the guiding hot path on one MI355X.
+
+Workload (full-frame wavefront size): hash grid 16 levels x 2^19 entries x 2
+features, OneBlob(16) over the 6 direction/normal/roughness inputs, MLP
+3 x 64 (bias-free, ReLU), K = 8 vMF lobes.
+  * infer + sample + pdf: 2^21 queries per step, each query encoded, run
+    through the network, mapped to its mixture (with the cosine product),
+    sampled once and evaluated at the sample and at 4 given directions;
+  * train step: 2^18 samples (forward, KL adjoint, MLP + grid backward, Adam).
+Inputs are synthetic (pcg32 keyed by query index) and resident in HBM before
+the timed region. One step of the headline metric is one launch of the fused
+query kernel, timed with HIP events on the stream the library launches on.
+
+Prints one JSON line (see the repository's bench contract).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neural-mixture-guiding_amd"))
+
+METRIC = "guided queries/s (infer+sample+pdf) & train samples/s at 2M batch, BF16 vs FP32"
+N_QUERIES = 1 << 21
+N_TRAIN = 1 << 18
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BYTES_PER_QUERY = 36 + 12 + 48 + 12 + 4 + 16 + 4  # in, u, 4 eval dirs; dir, pdf, 4 pdfs, lobe
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_region(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps  # ms per step
+
+
+def cpu_baseline(cfg, params, q_host, budget_s=10.0):
+    """The oracle's FP32 path on the host cores, on a bounded sample of the same queries."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+
+    o = pyoracle.Oracle()
+    oc = pyoracle.make_cfg(cfg)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = 32768
+    done, elapsed = 0, 0.0
+    while elapsed < budget_s and done + n <= N_QUERIES:
+        sl = slice(done, done + n)
+        args = [np.ascontiguousarray(q_host[k][..., sl]) for k in ("inputs", "normal", "u", "eval_dirs")]
+        t0 = time.perf_counter()
+        o.query(oc, params, *args, n_threads=threads)
+        elapsed += time.perf_counter() - t0
+        done += n
+        n = min(n * 2, 1 << 19)
+    return {"value": done / elapsed, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of the 2^21 benchmark queries (FP32 oracle, encode+MLP+mixture+sample+4 pdfs), "
+                      f"{elapsed:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the guiding path would shard queries by index, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import npmg
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    cfg = npmg.benchmark_config("full", precision=args.precision)
+    net = npmg.GuidingNetwork(cfg, seed=7272)
+    log(f"network: {net.n_mlp} MLP + {net.n_grid} grid parameters")
+    q = npmg.synth_queries(2024, N_QUERIES)
+    out = {
+        "dir": torch.empty(3, N_QUERIES, device="cuda"), "pdf": torch.empty(N_QUERIES, device="cuda"),
+        "eval_pdf": torch.empty(4, N_QUERIES, device="cuda"), "lobe": torch.empty(N_QUERIES, dtype=torch.int32, device="cuda"),
+    }
+    lib = npmg.lib()
+    P = npmg._ptr
+
+    def query_step():
+        st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
+                                 P(out["dir"]), P(out["pdf"]), P(out["eval_pdf"]), P(out["lobe"]), None,
+                                 N_QUERIES, npmg.NPMG_COSINE_AWARE, stream.cuda_stream)
+        if st != 0:
+            raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
+
+    ms_q = time_region(query_step, args.steps, args.warmup, stream)
+    qps = N_QUERIES / (ms_q * 1e-3)
+    log(f"infer+sample+pdf: {ms_q:.3f} ms per 2^21 queries ({qps / 1e6:.1f} M queries/s)")
+
+    t = npmg.synth_train(77, N_TRAIN)
+    loss = torch.empty(1, device="cuda")
+
+    def train_step():
+        st = lib.npmg_train_step(net._h, P(t["inputs"]), P(t["tgt_dir"]), P(t["tgt_rgb"]), P(t["tgt_pdf"]),
+                                 N_TRAIN, P(loss), stream.cuda_stream)
+        if st != 0:
+            raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
+
+    ms_t = time_region(train_step, max(2, args.steps // 2), 2, stream)
+    sps = N_TRAIN / (ms_t * 1e-3)
+    log(f"train step: {ms_t:.3f} ms per 2^18 samples ({sps / 1e6:.2f} M samples/s), loss {loss.item():.4f}")
+
+    table_bytes = net.n_grid * (4 if args.precision == "fp32" else 2)
+    alg_bytes = BYTES_PER_QUERY * N_QUERIES + table_bytes
+    achieved = alg_bytes / (ms_q * 1e-3) / 1e9
+    flops = 2 * net.n_mlp * N_QUERIES
+    result = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_q,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (pcg32-keyed hit records, analytic radiance field)",
+        "config": {"workload": "infer+sample+pdf of 2^21 queries, hash grid 16x2^19x2 + OneBlob(16)x6, "
+                               "MLP 3x64, 8 vMF lobes, cosine product, 4 eval directions",
+                   "queries_per_step": N_QUERIES, "train_batch": N_TRAIN, "precision": args.precision},
+        "train_samples_per_s": sps,
+        "train_ms_per_step": ms_t,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_query_fp32 (fused encode+MLP+mixture+sample+pdf)",
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "mlp_compute": {"flops_per_launch": flops, "achieved_tflops": flops / (ms_q * 1e-3) / 1e12,
+                        "peak_tflops": FP32_PEAK_TFLOPS},
+    }
+    if not args.no_cpu:
+        params = net.get_params(1).cpu().numpy()
+        q_host = {k: v.cpu().numpy() for k, v in q.items()}
+        result["cpu_baseline"] = cpu_baseline(cfg, params, q_host)
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.0f} queries/s")
+    net.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/* npmg.h -- C ABI of the MI355X neural parametric mixture guiding library.
+ *
+ * This boundary replaces the calls that the reference's guided integrator
+ * makes into tiny-cuda-nn and into the device-side mixture class:
+ *   - Network::inference             src/render/guided/integrator.cpp:645
+ *   - Network::forward / backward    src/render/guided/integrator.cpp:681, :694
+ *   - compute_dL_doutput_divergence  src/render/guided/integrator.cpp:689-691,
+ *                                    src/render/guided/train.cu:69-124
+ *   - Trainer::optimizer_step        src/render/guided/integrator.cpp:695
+ *   - thrust::reduce of the loss     src/render/guided/integrator.cpp:697
+ *   - MixedSphericalGaussianDistribution<8>{ctor, applyCosineLobe, sample, pdf}
+ *     in the consumer kernels        src/render/guided/integrator.cpp:198-201,
+ *                                    :257-260, :297-317
+ *   - resetNetwork / initialize_params src/render/guided/integrator.cpp:555-587
+ *
+ * Conventions
+ *   - Every pointer is a device pointer unless documented as host.
+ *   - Per-query data is structure-of-arrays: a field of d values is [d][n],
+ *     query index fastest (the reference uses column-major [dims][n] network
+ *     inputs, tiny-cuda-nn/object.h:147-169).
+ *   - n is arbitrary (the reference's multiple-of-128 rule is not required).
+ *   - Calls that take a stream enqueue asynchronously on it; a NULL stream is
+ *     the default stream. Create, destroy, parameter get/set and size queries
+ *     are synchronous.
+ *   - Errors are returned as npmg_status; no C++ exception crosses the ABI.
+ *     npmg_last_error() returns a message for the last failing call on a
+ *     handle (or, with NULL, for the last failing npmg_create).
+ *   - A handle is not thread-safe; use one handle per host thread.
+ */
+#ifndef NPMG_H
+#define NPMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct npmg_ctx npmg_ctx;
+typedef void* npmg_stream; /* hipStream_t */
+
+typedef enum {
+    NPMG_OK = 0,
+    NPMG_EINVAL = 1, /* bad argument or configuration */
+    NPMG_ENOMEM = 2, /* device allocation failed */
+    NPMG_EHIP = 3,   /* HIP runtime error */
+    NPMG_ESTATE = 4  /* call not valid in the current state */
+} npmg_status;
+
+/* sample/pdf flags */
+#define NPMG_COSINE_AWARE 1u /* multiply each lobe by the cosine vMF about `normal`
+                                (applyCosineLobe, distribution.h:184-195) */
+
+/* Creates a guiding network from a JSON configuration (host string). The
+ * "encoding", "network" and "optimizer" objects use tiny-cuda-nn's keys, so the
+ * reference's common/configs/nn/base.json "nn" object is accepted as is;
+ * "mixture", "loss", "precision" and "grid_gradient" select the build's modes.
+ * Replaces GuidedPathTracer::resetNetwork (integrator.cpp:555-587) including
+ * Trainer::initialize_params (trainer.h:53-92) with `seed` in place of 7272. */
+npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out);
+void npmg_destroy(npmg_ctx* ctx);
+const char* npmg_last_error(const npmg_ctx* ctx);
+
+/* Parameter counts: MLP weights first, then the grid table (GuidingNetwork
+ * parameter order, src/render/guided/network.h:85-100). */
+npmg_status npmg_n_params(const npmg_ctx* ctx, size_t* n_mlp, size_t* n_grid);
+/* Encoded width (padded to 16) and padded output width of the network. */
+npmg_status npmg_widths(const npmg_ctx* ctx, uint32_t* encoded_width, uint32_t* padded_output);
+/* Grid level table: offsets[n_levels+1] (entries), scales[n_levels], resolutions[n_levels]
+ * (tiny-cuda-nn encodings/grid.h:940-1004). Host arrays. */
+npmg_status npmg_grid_layout(const npmg_ctx* ctx, uint32_t* n_levels, uint32_t* offsets, float* scales,
+                             uint32_t* resolutions);
+
+/* Full-precision parameters in the tcnn order. set_params writes the training
+ * master copy and the inference (EMA) copy; which = 0 reads the training
+ * master, which = 1 the inference copy. Trainer::set_params_full_precision /
+ * params_inference (trainer.h:314-328). */
+npmg_status npmg_set_params(npmg_ctx* ctx, const float* params, size_t n, int on_device);
+npmg_status npmg_get_params(const npmg_ctx* ctx, float* params, size_t n, int on_device, int which);
+
+/* Network inference with the inference (EMA) weights: in [9][n] -> raw
+ * network outputs [n_out][n] (Network::inference, object.h:147-169). */
+npmg_status npmg_infer(npmg_ctx* ctx, const float* in, float* raw_out, uint32_t n, npmg_stream stream);
+
+/* Fused query path: encode -> MLP -> mixture -> optional cosine product ->
+ * one sample (u [3][n]: lobe choice then the 2D direction sample, the draw
+ * order of distribution.h:132-140) -> pdf of the sample and of n_eval given
+ * directions eval_dirs [n_eval][3][n]. normal [3][n] is the face-forwarded
+ * shading normal used with NPMG_COSINE_AWARE (may be NULL otherwise).
+ * Outputs: out_dir [3][n], out_pdf [n], out_eval_pdf [n_eval][n], out_lobe
+ * [n] (may be NULL). u may be NULL to skip sampling (then out_dir/out_pdf are
+ * unused). raw_out [n_out][n] optionally receives the network outputs. */
+npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal, const float* u,
+                            const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                            float* out_eval_pdf, uint32_t* out_lobe, float* raw_out, uint32_t n,
+                            uint32_t flags, npmg_stream stream);
+
+/* Consumer side only: the same mixture work from raw outputs [n_out][n]
+ * produced by npmg_infer (the three consumer lambdas of integrator.cpp:158-340). */
+npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float* normal, const float* u,
+                                    const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                                    float* out_eval_pdf, uint32_t* out_lobe, float* lobes_out, uint32_t n,
+                                    uint32_t flags, npmg_stream stream);
+
+/* One training step on a batch (integrator.cpp:679-699): forward with the
+ * training weights, KL loss and mixture adjoint (train.cu:69-124), MLP and
+ * grid backward, Adam (+EMA) with the context's loss scale, and the mean loss
+ * written to the device scalar loss_out (may be NULL). Targets: tgt_dir [2][n]
+ * (theta, phi), tgt_rgb [3][n], tgt_pdf [n] (GuidedOutput, guideditem.h:57-64).
+ * No host synchronisation. */
+npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                            const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream);
+
+/* ---- Split passes (test harness and autograd wrapping) ------------------ */
+
+/* Encoding only: in [9][n] -> enc [encoded_width][n]. */
+npmg_status npmg_encode(npmg_ctx* ctx, const float* in, float* enc_out, uint32_t n, int use_inference_params,
+                        npmg_stream stream);
+/* Per (level, corner) hash-grid table index relative to the level and
+ * trilinear weight: idx [n_levels][8][n], w [n_levels][8][n]. */
+npmg_status npmg_grid_corners(npmg_ctx* ctx, const float* in, uint32_t* idx, float* w, uint32_t n,
+                              npmg_stream stream);
+/* Forward with the training (0) or inference (1) weights: raw [n_out][n]. */
+npmg_status npmg_forward(npmg_ctx* ctx, const float* in, float* raw_out, uint32_t n, int use_inference_params,
+                         npmg_stream stream);
+/* Loss and adjoint w.r.t. the raw outputs, scaled by loss_scale / n like the
+ * reference: dout [n_out][n], loss [n] (per-sample -f/q log p). */
+npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                                 const float* tgt_pdf, uint32_t n, float loss_scale, float* dout, float* loss,
+                                 npmg_stream stream);
+/* Backward of the network with the training weights given dL/draw [n_out][n]:
+ * writes the gradient of every parameter (tcnn order) to dparams [n_params]
+ * and, if denc is not NULL, dL/denc [encoded_width][n]. */
+npmg_status npmg_backward(npmg_ctx* ctx, const float* in, const float* dout, uint32_t n, float* dparams,
+                          float* denc, npmg_stream stream);
+/* Applies Adam (+EMA) to the training parameters with gradients dparams
+ * [n_params] scaled by loss_scale (Trainer::optimizer_step, trainer.h:153-155). */
+npmg_status npmg_optimizer_step(npmg_ctx* ctx, const float* dparams, float loss_scale, npmg_stream stream);
+/* Current loss scale and optimizer step count (host outputs). */
+npmg_status npmg_train_state(const npmg_ctx* ctx, float* loss_scale, uint32_t* step);
+
+/* ---- Synthetic wavefront records (hit records are synthetic) ------------ */
+
+/* Queries keyed by (seed, first + i) with pcg32: in [9][n] packed like
+ * generate_inference_data (train.cu:174-191), face-forwarded normal [3][n],
+ * sample uniforms u [3][n], eval_dirs [4][3][n] uniform on the sphere. */
+npmg_status npmg_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,
+                               float* eval_dirs, npmg_stream stream);
+/* Training records: cosine-weighted direction about the face-forwarded
+ * normal, its pdf, and the radiance of an analytic field of three vMF lobes
+ * whose axes rotate with position (the contract of generate_training_data,
+ * train.cu:14-67). */
+npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* in, float* tgt_dir, float* tgt_rgb,
+                             float* tgt_pdf, npmg_stream stream);
+
+/* Evaluates the library's device elementary functions over x (and y):
+ * fn 0 exp, 1 log, 2 log1p, 3 expm1, 4 sin, 5 cos, 6 acos, 7 atan2(x, y),
+ * 8 sqrt, 9 logistic, 10 x / y. For bitwise host/device comparisons. */
+npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPMG_H */

@@ -1,0 +1,535 @@
+// api.cpp -- the C ABI of include/npmg.h: context, parameter state and the
+// host-side sequencing of the kernels.
+//
+// The context owns every device buffer: full-precision parameters (tcnn
+// order: MLP matrices, then the grid table), the inference copy (EMA),
+// Adam moments and per-parameter step counters, gradients, the kernel-layout
+// (transposed) MLP weights for training and inference, and a training
+// workspace grown on demand. Calls enqueue on the caller's stream and never
+// synchronise the host, except create/destroy/get/set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/npmg.h"
+#include "config.h"
+#include "kernels.h"
+#include "npmg_math.h"
+
+namespace npmg {
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
+size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
+}  // namespace npmg
+
+using namespace npmg;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) return nullptr;
+    const hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw std::bad_alloc();
+    return static_cast<T*>(p);
+}
+
+// PCG32 with jump-ahead (tiny-cuda-nn dependencies/pcg32/pcg32.h:44-170).
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    explicit Pcg32(uint64_t initstate, uint64_t initseq = 1u) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next_uint();
+        state += initstate;
+        next_uint();
+    }
+    uint32_t next_uint() {
+        const uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31));
+    }
+    float next_float() { return fm::bits_to_float((next_uint() >> 9) | 0x3f800000u) - 1.0f; }
+    void advance(uint64_t delta) {
+        uint64_t cur_mult = 0x5851f42d4c957f2dULL, cur_plus = inc, acc_mult = 1u, acc_plus = 0u;
+        while (delta > 0) {
+            if (delta & 1) {
+                acc_mult *= cur_mult;
+                acc_plus = acc_plus * cur_mult + cur_plus;
+            }
+            cur_plus = (cur_mult + 1) * cur_plus;
+            cur_mult *= cur_mult;
+            delta /= 2;
+        }
+        state = acc_mult * state + acc_plus;
+    }
+};
+
+// Trainer's initialisation (tiny-cuda-nn trainer.h:53-92): std::seed_seq{seed}
+// -> pcg32; Xavier-uniform MLP matrices drawn in order on the host
+// (gpu_matrix.h:291-306); the grid table U(-1e-4, 1e-4) as generated by
+// generate_random_uniform's 4-values-per-thread kernel (random.h:66-97,
+// grid.h:1333-1336), after which the generator advances by the table size.
+void init_params(const Config& cfg, uint32_t seed, std::vector<float>& p) {
+    const DevCfg& d = cfg.d;
+    std::seed_seq seq{seed};
+    std::vector<uint32_t> seeds(2);
+    seq.generate(seeds.begin(), seeds.end());
+    Pcg32 rng(seeds.front());
+    p.assign(cfg.n_params, 0.0f);
+    for (int pass = 0; pass < std::max(1, cfg.init_passes); ++pass) {
+        for (int l = 0; l < d.n_layers; ++l) {
+            const float scale = std::sqrt(6.0f / (float)(d.layer_in[l] + d.layer_out[l]));
+            const size_t ne = (size_t)d.layer_in[l] * d.layer_out[l];
+            float* w = p.data() + d.layer_off[l];
+            for (size_t i = 0; i < ne; ++i) w[i] = rng.next_float() * 2.0f * scale - scale;
+        }
+        if (cfg.n_grid) {
+            const size_t n = cfg.n_grid, n_threads = (n + 3) / 4;
+            Pcg32 r = rng;
+            float* out = p.data() + cfg.n_mlp;
+            for (size_t k = 0; k < 4 * n_threads; ++k) {
+                const float val = r.next_float();
+                const size_t idx = k / 4 + n_threads * (k % 4);
+                if (idx < n) out[idx] = fm::fma_(val, 1e-4f - -1e-4f, -1e-4f);
+            }
+            rng.advance(n);
+        }
+    }
+}
+
+}  // namespace
+
+struct npmg_ctx {
+    Config cfg;
+    std::string err;
+    int n_cu = 256;
+    float* master = nullptr;  // training parameters, full precision
+    float* ema = nullptr;     // inference parameters (EMA), or null without EMA
+    float* m = nullptr;
+    float* v = nullptr;
+    uint32_t* steps = nullptr;
+    float* grad = nullptr;
+    float* wt_train = nullptr;  // transposed MLP weights
+    float* wt_infer = nullptr;
+    // training workspace
+    uint32_t ws_cap = 0;
+    TrainWS ws{};
+    float* partials = nullptr;
+    size_t partials_cap = 0;
+    uint32_t step = 0;
+    float loss_scale = 128.0f;
+
+    const float* infer_params() const { return ema ? ema : master; }
+
+    ~npmg_ctx() {
+        for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
+                        (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
+                        (void*)partials})
+            if (p) (void)hipFree(p);
+    }
+
+    void check_fused() const {
+        if (cfg.precision != PREC_FP32) throw std::invalid_argument("only precision fp32 is available in this build");
+        if (cfg.d.K != 8 && cfg.d.K != 16) throw std::invalid_argument("the fused kernels support 8 or 16 lobes");
+    }
+
+    void refresh_layouts(hipStream_t s) {
+        hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
+        hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
+    }
+
+    void ensure_ws(uint32_t n) {
+        if (n <= ws_cap) return;
+        for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss})
+            if (p) hip_check(hipFree(p), "hipFree");
+        ws = TrainWS{};
+        ws_cap = 0;
+        const DevCfg& d = cfg.d;
+        ws.enc = dalloc<float>((size_t)d.enc_padded * n);
+        ws.acts = dalloc<float>((size_t)d.n_hidden * d.width * n);
+        ws.g = dalloc<float>((size_t)d.n_hidden * d.width * n);
+        ws.dout = dalloc<float>((size_t)d.n_out_pad * n);
+        ws.loss = dalloc<float>(n);
+        uint32_t splits, qps;
+        const size_t need = wgrad_partials_needed(d, n, &splits, &qps);
+        if (need > partials_cap) {
+            if (partials) hip_check(hipFree(partials), "hipFree");
+            partials = dalloc<float>(need);
+            partials_cap = need;
+        }
+        ws_cap = n;
+    }
+
+    // Forward with the training weights keeping activations (+ adjoint when targets are given).
+    void train_forward(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, float ls,
+                       hipStream_t s) {
+        ensure_ws(n);
+        hip_check(launch_train_forward_fp32(cfg.d, wt_train, master + cfg.n_mlp, in, td, tr, tp, ls / (float)n, ws, n,
+                                            n_cu, s),
+                  "train forward");
+    }
+
+    // Backward from ws.dout: parameter gradients into dparams (tcnn order).
+    void train_backward(const float* in, float* dparams, float* denc, uint32_t n, hipStream_t s) {
+        if (cfg.n_grid) hip_check(launch_fill(dparams + cfg.n_mlp, 0.0f, cfg.n_grid, s), "fill");
+        hip_check(launch_train_backward_fp32(cfg.d, master, in, ws, denc, cfg.n_grid ? dparams + cfg.n_mlp : nullptr,
+                                             n, n_cu, s),
+                  "train backward");
+        hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, s), "weight gradient");
+    }
+
+    void optimizer_step(const float* dparams, float ls, hipStream_t s) {
+        ++step;
+        AdamArgs a{};
+        a.n = cfg.n_params;
+        a.n_matrix = cfg.n_mlp;
+        a.loss_scale = ls;
+        a.lr = cfg.lr;
+        a.beta1 = cfg.beta1;
+        a.beta2 = cfg.beta2;
+        a.eps = cfg.eps;
+        a.l2_reg = cfg.l2_reg;
+        a.ema = ema != nullptr;
+        a.ema_decay = cfg.ema_decay;
+        a.ema_debias_old = 1 - (float)std::pow(cfg.ema_decay, step - 1);
+        a.ema_debias_new = 1.0f / (1 - (float)std::pow(cfg.ema_decay, step));
+        hip_check(launch_adam(a, master, dparams, m, v, steps, ema, s), "adam");
+        refresh_layouts(s);
+    }
+};
+
+namespace {
+
+template <class F>
+npmg_status guarded(npmg_ctx* ctx, F&& f) {
+    try {
+        f();
+        return NPMG_OK;
+    } catch (const HipError& e) {
+        if (ctx) ctx->err = e.what();
+        return NPMG_EHIP;
+    } catch (const StateError& e) {
+        if (ctx) ctx->err = e.what();
+        return NPMG_ESTATE;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->err = "device or host allocation failed";
+        return NPMG_ENOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return NPMG_EINVAL;
+    } catch (...) {
+        if (ctx) ctx->err = "unknown error";
+        return NPMG_EINVAL;
+    }
+}
+
+hipStream_t S(npmg_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+void need(bool ok, const char* msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) {
+    g_create_error.clear();
+    if (!out || !json_config) {
+        g_create_error = "null argument";
+        return NPMG_EINVAL;
+    }
+    *out = nullptr;
+    npmg_ctx* ctx = new (std::nothrow) npmg_ctx();
+    if (!ctx) return NPMG_ENOMEM;
+    const npmg_status st = guarded(ctx, [&] {
+        ctx->cfg = parse_config(json_config);
+        const int K = ctx->cfg.d.K;
+        need(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, "n_lobes must be 1, 2, 4, 8 or 16");
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        const Config& c = ctx->cfg;
+        ctx->loss_scale = c.loss_scale;
+        ctx->master = dalloc<float>(c.n_params);
+        if (c.ema) ctx->ema = dalloc<float>(c.n_params);
+        ctx->m = dalloc<float>(c.n_params);
+        ctx->v = dalloc<float>(c.n_params);
+        ctx->steps = dalloc<uint32_t>(c.n_params);
+        ctx->grad = dalloc<float>(c.n_params);
+        ctx->wt_train = dalloc<float>(c.n_mlp);
+        ctx->wt_infer = dalloc<float>(c.n_mlp);
+        hip_check(hipMemset(ctx->m, 0, c.n_params * sizeof(float)), "memset");
+        hip_check(hipMemset(ctx->v, 0, c.n_params * sizeof(float)), "memset");
+        hip_check(hipMemset(ctx->steps, 0, c.n_params * sizeof(uint32_t)), "memset");
+        hip_check(hipMemset(ctx->grad, 0, c.n_params * sizeof(float)), "memset");
+        std::vector<float> p;
+        init_params(c, (uint32_t)seed, p);
+        hip_check(hipMemcpy(ctx->master, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+        if (ctx->ema) hip_check(hipMemcpy(ctx->ema, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+        ctx->refresh_layouts(nullptr);
+        hip_check(hipDeviceSynchronize(), "synchronize");
+    });
+    if (st != NPMG_OK) {
+        g_create_error = ctx->err;
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return NPMG_OK;
+}
+
+void npmg_destroy(npmg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+const char* npmg_last_error(const npmg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+npmg_status npmg_n_params(const npmg_ctx* ctx, size_t* n_mlp, size_t* n_grid) {
+    if (!ctx || !n_mlp || !n_grid) return NPMG_EINVAL;
+    *n_mlp = ctx->cfg.n_mlp;
+    *n_grid = ctx->cfg.n_grid;
+    return NPMG_OK;
+}
+
+npmg_status npmg_widths(const npmg_ctx* ctx, uint32_t* encoded_width, uint32_t* padded_output) {
+    if (!ctx || !encoded_width || !padded_output) return NPMG_EINVAL;
+    *encoded_width = (uint32_t)ctx->cfg.d.enc_padded;
+    *padded_output = (uint32_t)ctx->cfg.d.n_out_pad;
+    return NPMG_OK;
+}
+
+npmg_status npmg_grid_layout(const npmg_ctx* ctx, uint32_t* n_levels, uint32_t* offsets, float* scales,
+                             uint32_t* resolutions) {
+    if (!ctx || !n_levels) return NPMG_EINVAL;
+    const DevCfg& d = ctx->cfg.d;
+    *n_levels = d.grid_enc >= 0 ? (uint32_t)d.n_levels : 0u;
+    if (d.grid_enc < 0) return NPMG_OK;
+    for (int l = 0; l <= d.n_levels; ++l) if (offsets) offsets[l] = d.offsets[l];
+    for (int l = 0; l < d.n_levels; ++l) {
+        if (scales) scales[l] = d.scales[l];
+        if (resolutions) resolutions[l] = d.res[l];
+    }
+    return NPMG_OK;
+}
+
+npmg_status npmg_set_params(npmg_ctx* ctx, const float* params, size_t n, int on_device) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(params && n == ctx->cfg.n_params, "params must hold exactly n_params values");
+        const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        hip_check(hipMemcpy(ctx->master, params, n * sizeof(float), k), "copy params");
+        if (ctx->ema) hip_check(hipMemcpy(ctx->ema, params, n * sizeof(float), k), "copy params");
+        ctx->refresh_layouts(nullptr);
+        hip_check(hipDeviceSynchronize(), "synchronize");
+    });
+}
+
+npmg_status npmg_get_params(const npmg_ctx* ctx, float* params, size_t n, int on_device, int which) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(const_cast<npmg_ctx*>(ctx), [&] {
+        need(params && n == ctx->cfg.n_params, "params must hold exactly n_params values");
+        need(which == 0 || which == 1, "which must be 0 (training) or 1 (inference)");
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        const float* src = which == 0 ? ctx->master : ctx->infer_params();
+        hip_check(hipMemcpy(params, src, n * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
+                  "copy params");
+    });
+}
+
+npmg_status npmg_infer(npmg_ctx* ctx, const float* in, float* raw_out, uint32_t n, npmg_stream stream) {
+    return npmg_forward(ctx, in, raw_out, n, 1, stream);
+}
+
+npmg_status npmg_forward(npmg_ctx* ctx, const float* in, float* raw_out, uint32_t n, int use_inference_params,
+                         npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n == 0 || (in && raw_out), "null buffer");
+        QueryIO io{};
+        io.in = in;
+        io.raw_out = raw_out;
+        const float* wt = use_inference_params ? ctx->wt_infer : ctx->wt_train;
+        const float* p = use_inference_params ? ctx->infer_params() : ctx->master;
+        hip_check(launch_query_fp32(ctx->cfg.d, wt, p + ctx->cfg.n_mlp, io, n, ctx->n_cu, S(stream)), "forward");
+    });
+}
+
+npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal, const float* u,
+                            const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                            float* out_eval_pdf, uint32_t* out_lobe, float* raw_out, uint32_t n, uint32_t flags,
+                            npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n == 0 || in, "null input");
+        need(!u || (out_dir && out_pdf), "sampling needs out_dir and out_pdf");
+        need(n_eval == 0 || (eval_dirs && out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
+        need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
+        QueryIO io{};
+        io.in = in;
+        io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
+        io.u = u;
+        io.eval_dirs = eval_dirs;
+        io.n_eval = (int)n_eval;
+        io.out_dir = out_dir;
+        io.out_pdf = out_pdf;
+        io.out_eval_pdf = out_eval_pdf;
+        io.out_lobe = out_lobe;
+        io.raw_out = raw_out;
+        hip_check(launch_query_fp32(ctx->cfg.d, ctx->wt_infer, ctx->infer_params() + ctx->cfg.n_mlp, io, n, ctx->n_cu,
+                                    S(stream)),
+                  "query");
+    });
+}
+
+npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float* normal, const float* u,
+                                    const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                                    float* out_eval_pdf, uint32_t* out_lobe, float* lobes_out, uint32_t n,
+                                    uint32_t flags, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n == 0 || raw, "null raw outputs");
+        need(!u || (out_dir && out_pdf), "sampling needs out_dir and out_pdf");
+        need(n_eval == 0 || (eval_dirs && out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
+        need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
+        need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the mixture kernel supports 8 or 16 lobes");
+        QueryIO io{};
+        io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
+        io.u = u;
+        io.eval_dirs = eval_dirs;
+        io.n_eval = (int)n_eval;
+        io.out_dir = out_dir;
+        io.out_pdf = out_pdf;
+        io.out_eval_pdf = out_eval_pdf;
+        io.out_lobe = out_lobe;
+        io.lobes_out = lobes_out;
+        hip_check(launch_mixture(ctx->cfg.d, raw, io, n, S(stream)), "mixture");
+    });
+}
+
+npmg_status npmg_encode(npmg_ctx* ctx, const float* in, float* enc_out, uint32_t n, int use_inference_params,
+                        npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n == 0 || (in && enc_out), "null buffer");
+        const float* p = use_inference_params ? ctx->infer_params() : ctx->master;
+        hip_check(launch_encode(ctx->cfg.d, p + ctx->cfg.n_mlp, in, enc_out, n, S(stream)), "encode");
+    });
+}
+
+npmg_status npmg_grid_corners(npmg_ctx* ctx, const float* in, uint32_t* idx, float* w, uint32_t n,
+                              npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(ctx->cfg.d.grid_enc >= 0, "configuration has no grid encoding");
+        need(n == 0 || (in && idx && w), "null buffer");
+        hip_check(launch_grid_corners(ctx->cfg.d, in, idx, w, n, S(stream)), "grid corners");
+    });
+}
+
+npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                                 const float* tgt_pdf, uint32_t n, float loss_scale, float* dout, float* loss,
+                                 npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && dout && loss), "null buffer");
+        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf, n ? loss_scale / (float)n : 0.0f, dout,
+                                 loss, n, S(stream)),
+                  "adjoint");
+    });
+}
+
+npmg_status npmg_backward(npmg_ctx* ctx, const float* in, const float* dout, uint32_t n, float* dparams, float* denc,
+                          npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n > 0 && in && dout && dparams, "null buffer or empty batch");
+        ctx->train_forward(in, nullptr, nullptr, nullptr, n, 1.0f, S(stream));
+        hip_check(hipMemcpyAsync(ctx->ws.dout, dout, (size_t)ctx->cfg.d.n_out_pad * n * sizeof(float),
+                                 hipMemcpyDeviceToDevice, S(stream)),
+                  "copy dout");
+        ctx->train_backward(in, dparams, denc, n, S(stream));
+    });
+}
+
+npmg_status npmg_optimizer_step(npmg_ctx* ctx, const float* dparams, float loss_scale, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(dparams != nullptr, "null gradients");
+        need(loss_scale > 0, "loss_scale must be positive");
+        ctx->optimizer_step(dparams, loss_scale, S(stream));
+    });
+}
+
+npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                            const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
+        hipStream_t s = S(stream);
+        const float ls = ctx->loss_scale;
+        ctx->train_forward(in, tgt_dir, tgt_rgb, tgt_pdf, n, ls, s);
+        ctx->train_backward(in, ctx->grad, nullptr, n, s);
+        ctx->optimizer_step(ctx->grad, ls, s);
+        if (loss_out) hip_check(launch_mean(ctx->ws.loss, n, loss_out, s), "loss mean");
+    });
+}
+
+npmg_status npmg_train_state(const npmg_ctx* ctx, float* loss_scale, uint32_t* step) {
+    if (!ctx) return NPMG_EINVAL;
+    if (loss_scale) *loss_scale = ctx->loss_scale;
+    if (step) *step = ctx->step;
+    return NPMG_OK;
+}
+
+npmg_status npmg_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,
+                               float* eval_dirs, npmg_stream stream) {
+    if (n && !(in && normal && u && eval_dirs)) return NPMG_EINVAL;
+    return launch_synth_queries(seed, first, n, in, normal, u, eval_dirs, S(stream)) == hipSuccess ? NPMG_OK
+                                                                                                   : NPMG_EHIP;
+}
+
+npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* in, float* tgt_dir, float* tgt_rgb,
+                             float* tgt_pdf, npmg_stream stream) {
+    if (n && !(in && tgt_dir && tgt_rgb && tgt_pdf)) return NPMG_EINVAL;
+    const FieldParams f = make_field(seed);
+    return launch_synth_train(seed, first, n, f, in, tgt_dir, tgt_rgb, tgt_pdf, S(stream)) == hipSuccess ? NPMG_OK
+                                                                                                       : NPMG_EHIP;
+}
+
+npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream) {
+    if (n && !(x && out)) return NPMG_EINVAL;
+    if (fn < 0 || fn > 10) return NPMG_EINVAL;
+    return launch_debug_math(fn, x, y, n, out, S(stream)) == hipSuccess ? NPMG_OK : NPMG_EHIP;
+}
+
+}  // extern "C"

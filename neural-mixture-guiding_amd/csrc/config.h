@@ -1,0 +1,62 @@
+// config.h -- network configuration (host) and its kernel-argument image.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace npmg {
+
+enum EncType : int { ENC_GRID = 0, ENC_ONEBLOB = 1, ENC_SH = 2, ENC_IDENTITY = 3 };
+enum GridType : int { GRID_HASH = 0, GRID_DENSE = 1 };
+enum HashType : int { HASH_COHERENT_PRIME = 0, HASH_PRIME = 1, HASH_REVERSED_PRIME = 2 };
+enum Interp : int { INTERP_LINEAR = 0, INTERP_SMOOTHSTEP = 1, INTERP_NEAREST = 2 };
+enum Mapping : int { MAP_REFERENCE = 0, MAP_SOFTMAX = 1 };
+enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1 };
+enum MixtureType : int { MIX_VMF = 0 };
+enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3 };
+
+constexpr int kMaxEnc = 4;
+constexpr int kMaxLevels = 32;
+constexpr int kMaxLayers = 9;
+constexpr int kMaxLobes = 16;
+
+// Everything a kernel needs to know about the network, passed by value.
+struct DevCfg {
+    int n_in;  // input dims of a query record (9)
+    int n_enc;
+    int enc_type[kMaxEnc], enc_begin[kMaxEnc], enc_ndims[kMaxEnc], enc_param[kMaxEnc], enc_offset[kMaxEnc];
+    int enc_width, enc_padded;
+    int grid_enc;  // index of the grid encoding, -1 if none
+    int grid_type, hash_type, n_levels, F, interp;
+    uint32_t offsets[kMaxLevels + 1];
+    float scales[kMaxLevels];
+    uint32_t res[kMaxLevels];
+    int width, n_hidden, n_layers, n_out, n_out_pad;
+    int layer_in[kMaxLayers], layer_out[kMaxLayers];
+    uint32_t layer_off[kMaxLayers];  // offset of each weight matrix in the parameter vector
+    int K, mapping, numerics, mixture;
+    float cos_kappa, grad_clamp;
+};
+
+struct Config {
+    DevCfg d{};
+    Precision precision = PREC_FP32;
+    // Adam (tiny-cuda-nn optimizers/adam.h) and EMA (optimizers/ema.h)
+    float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, l2_reg = 1e-6f;
+    bool ema = true;
+    float ema_decay = 0.95f;
+    // loss scale: static (reference, parameters.h:24) or dynamic
+    float loss_scale = 128.0f;
+    bool dynamic_scale = false;
+    uint32_t scale_window = 1000;
+    bool deterministic_grid = false;
+    int init_passes = 2;
+    size_t n_mlp = 0, n_grid = 0, n_params = 0;
+};
+
+// Parses the JSON configuration; throws std::runtime_error on invalid input.
+Config parse_config(const std::string& json);
+
+}  // namespace npmg

@@ -1,0 +1,534 @@
+// device.h -- device building blocks of the guiding path: pcg32, hash-grid
+// corner indexing, OneBlob/SH encodings, and the vMF mixture (map, cosine
+// product, pdf, sample, adjoint).
+//
+// Floating-point expressions are written in a fixed order and the library is
+// built with -ffp-contract=off; explicit fma marks the accumulations that the
+// reference's nvcc build contracts. Together with npmg_math.h this makes the
+// FP32 path reproduce the CPU oracle bit for bit (indices, lobe selection,
+// encoded features, network outputs and mixture values).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "config.h"
+#include "npmg_math.h"
+
+namespace npmg {
+namespace dev {
+
+namespace fm = ::npmg::fm;
+
+// Constants of krrmath (src/backend/src/core/math/include/krrmath/constants.h:15-25).
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647693f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kInv2Pi = 0.15915494309189533577f;
+constexpr float kInv4Pi = 0.07957747154594766788f;
+constexpr float kEps = 1e-5f;  // M_EPSILON
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float a, float b, float c) { return {a, b, c}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return fm::sqrt_(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return divs(a, length(a)); }
+// std::min / std::max semantics (second argument wins only when strictly smaller / larger).
+__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float clampv(float v, float lo, float hi) { return smin(smax(v, lo), hi); }
+
+// math_utils.h:191-201
+__device__ __forceinline__ V3 sph_to_cart(float theta, float phi) {
+    float st, ct, sp, cp;
+    fm::sincos(theta, &st, &ct);
+    fm::sincos(phi, &sp, &cp);
+    return {st * cp, st * sp, ct};
+}
+// math_utils.h:176-183
+__device__ __forceinline__ void cart_to_sph(V3 v, float* theta, float* phi) {
+    const V3 vn = normalize(v);
+    *theta = fm::acos(vn.z);
+    float p = fm::atan2(vn.y, vn.x);
+    if (p < 0) p += kTwoPi;
+    *phi = p;
+}
+// math_utils.h:141-151
+__device__ __forceinline__ V3 get_perpendicular(V3 u) {
+    const float ax = fm::abs_(u.x), ay = fm::abs_(u.y), az = fm::abs_(u.z);
+    const uint32_t uyx = (ax - ay) < 0 ? 1u : 0u;
+    const uint32_t uzx = (ax - az) < 0 ? 1u : 0u;
+    const uint32_t uzy = (ay - az) < 0 ? 1u : 0u;
+    const uint32_t xm = uyx & uzx;
+    const uint32_t ym = (1u ^ xm) & uzy;
+    const uint32_t zm = 1u ^ (xm | ym);
+    return normalize(cross(u, mk((float)xm, (float)ym, (float)zm)));
+}
+// raytracing.h:35-56: Frame(n).toWorld(v)
+__device__ __forceinline__ V3 frame_to_world(V3 n, V3 v) {
+    const V3 t = get_perpendicular(n);
+    const V3 b = normalize(cross(n, t));
+    return add(add(mul(t, v.x), mul(b, v.y)), mul(n, v.z));
+}
+// sampling.h:32-37
+__device__ __forceinline__ V3 uniform_sample_sphere(float u0, float u1) {
+    const float z = 1.0f - 2.0f * u0;
+    const float r = fm::sqrt_(smax(0.0f, 1.0f - z * z));
+    float s, c;
+    fm::sincos(kTwoPi * u1, &s, &c);
+    return {r * c, r * s, z};
+}
+
+// ---------------------------------------------------------------------------
+// PCG32 (O'Neill 2014) with a per-query stream, for synthetic records.
+struct Pcg32 {
+    uint64_t state, inc;
+    __device__ __forceinline__ Pcg32(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next_uint();
+        state += initstate;
+        next_uint();
+    }
+    __device__ __forceinline__ uint32_t next_uint() {
+        const uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    __device__ __forceinline__ float next_float() {
+        return fm::bits_to_float((next_uint() >> 9) | 0x3f800000u) - 1.0f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Hash grid (tiny-cuda-nn encodings/grid.h:111-187, common_device.h:470-496).
+__device__ __forceinline__ uint32_t grid_index(const DevCfg& c, uint32_t hsize, uint32_t res, uint32_t p0, uint32_t p1,
+                                               uint32_t p2) {
+    uint32_t stride = 1, index = 0;
+    const uint32_t p[3] = {p0, p1, p2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (stride <= hsize) {
+            index += p[d] * stride;
+            stride *= res;
+        }
+    }
+    if (c.grid_type == GRID_HASH && hsize < stride) {
+        uint32_t f0 = 1u, f1 = 2654435761u, f2 = 805459861u;
+        if (c.hash_type == HASH_PRIME) f0 = 1958374283u;
+        else if (c.hash_type == HASH_REVERSED_PRIME) { f0 = 2165219737u; f1 = 1434869437u; f2 = 2097192037u; }
+        index = (p0 * f0) ^ (p1 * f1) ^ (p2 * f2);
+    }
+    return index % hsize;
+}
+
+// Fractional position and cell of one coordinate; the reference's nvcc build
+// contracts input * scale + 0.5 into one fma.
+__device__ __forceinline__ void pos_fract(float x, float scale, int interp, float* frac, uint32_t* cell) {
+    const float p = fm::fma_(x, scale, 0.5f);
+    const float fl = fm::floor_(p);
+    *cell = (uint32_t)(int)fl;
+    float f = p - fl;
+    if (interp == INTERP_SMOOTHSTEP) f = f * f * (3.0f - 2.0f * f);
+    *frac = f;
+}
+
+// The 8 corners of one level: index relative to the level and weight.
+// Weights are products over dims in order x, y, z of (1 - frac) or frac.
+__device__ __forceinline__ int level_corners(const DevCfg& c, int l, float x0, float x1, float x2, uint32_t idx[8],
+                                             float w[8]) {
+    const uint32_t hsize = c.offsets[l + 1] - c.offsets[l];
+    const float scale = c.scales[l];
+    const uint32_t res = c.res[l];
+    float f0, f1, f2;
+    uint32_t c0, c1, c2;
+    pos_fract(x0, scale, c.interp, &f0, &c0);
+    pos_fract(x1, scale, c.interp, &f1, &c1);
+    pos_fract(x2, scale, c.interp, &f2, &c2);
+    if (c.interp == INTERP_NEAREST) {
+        idx[0] = grid_index(c, hsize, res, c0, c1, c2);
+        w[0] = 1.0f;
+        return 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float wk = 1.0f;
+        wk *= (k & 1) ? f0 : 1.0f - f0;
+        wk *= (k & 2) ? f1 : 1.0f - f1;
+        wk *= (k & 4) ? f2 : 1.0f - f2;
+        idx[k] = grid_index(c, hsize, res, c0 + ((k & 1) ? 1u : 0u), c1 + ((k & 2) ? 1u : 0u), c2 + ((k & 4) ? 1u : 0u));
+        w[k] = wk;
+    }
+    return 8;
+}
+
+// tiny-cuda-nn common_device.h:533-548
+__device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
+    const float u = x * inv_radius;
+    const float u2 = u * u;
+    const float u4 = u2 * u2;
+    const float v = 15.0f / 16.0f * u * (1.0f - 2.0f / 3.0f * u2 + 1.0f / 5.0f * u4) + 0.5f;
+    return smax(0.0f, smin(1.0f, v));
+}
+
+// q_in[d] for a runtime d without indexing private memory (keeps q_in in VGPRs).
+__device__ __forceinline__ float pick(const float* q_in, int d) {
+    float v = q_in[0];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v = d == k ? q_in[k] : v;
+    return v;
+}
+
+// Feeds every encoded feature of one query, in order, to `feed(value)`:
+// grid levels (tiny-cuda-nn kernel_grid), OneBlob bins (kernel_one_blob_soa,
+// oneblob.h:98-127), SH (spherical_harmonics.h:45-63), identity, then the
+// 1.0 padding of the composite to a multiple of 16.
+template <class T, class Feed>
+__device__ __forceinline__ void encode_query(const DevCfg& c, const T* __restrict__ table, const float* q_in,
+                                             Feed&& feed) {
+    for (int e = 0; e < c.n_enc; ++e) {
+        const int b = c.enc_begin[e];
+        const int type = c.enc_type[e];
+        if (type == ENC_GRID) {
+            const float x0 = pick(q_in, b), x1 = pick(q_in, b + 1), x2 = pick(q_in, b + 2);
+            for (int l = 0; l < c.n_levels; ++l) {
+                uint32_t idx[8];
+                float w[8];
+                const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+                const T* tl = table + (size_t)c.offsets[l] * c.F;
+                if (c.F == 2) {
+                    float a0 = 0.0f, a1 = 0.0f;
+                    if (nc == 1) {
+                        a0 = (float)tl[idx[0] * 2u];
+                        a1 = (float)tl[idx[0] * 2u + 1u];
+                    } else {
+                        T v[16];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            v[2 * k] = tl[idx[k] * 2u];
+                            v[2 * k + 1] = tl[idx[k] * 2u + 1u];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            a0 = fm::fma_(w[k], (float)v[2 * k], a0);
+                            a1 = fm::fma_(w[k], (float)v[2 * k + 1], a1);
+                        }
+                    }
+                    feed(a0);
+                    feed(a1);
+                } else {
+                    for (int f = 0; f < c.F; ++f) {
+                        float a = 0.0f;
+                        if (nc == 1) {
+                            a = (float)tl[idx[0] * (uint32_t)c.F + f];
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) a = fm::fma_(w[k], (float)tl[idx[k] * (uint32_t)c.F + f], a);
+                        }
+                        feed(a);
+                    }
+                }
+            }
+        } else if (type == ENC_ONEBLOB) {
+            const int nb = c.enc_param[e];
+            const float fnb = (float)nb;
+            for (int d = 0; d < c.enc_ndims[e]; ++d) {
+                const float xv = pick(q_in, b + d);
+                float left = quartic_cdf(-xv, fnb) + quartic_cdf(-xv - 1.0f, fnb) + quartic_cdf(-xv + 1.0f, fnb);
+                for (int k = 0; k < nb; ++k) {
+                    const float rb = (float)(k + 1) / fnb;
+                    const float right =
+                        quartic_cdf(rb - xv, fnb) + quartic_cdf(rb - xv - 1.0f, fnb) + quartic_cdf(rb - xv + 1.0f, fnb);
+                    feed(right - left);
+                    left = right;
+                }
+            }
+        } else if (type == ENC_SH) {
+            const float sx = pick(q_in, b) * 2.0f - 1.0f, sy = pick(q_in, b + 1) * 2.0f - 1.0f,
+                        sz = pick(q_in, b + 2) * 2.0f - 1.0f;
+            feed(0.28209479177387814f);
+            if (c.enc_param[e] > 1) {
+                feed(-0.48860251190291987f * sy);
+                feed(0.48860251190291987f * sz);
+                feed(-0.48860251190291987f * sx);
+            }
+        } else {
+            for (int d = 0; d < c.enc_ndims[e]; ++d) feed(pick(q_in, b + d));
+        }
+    }
+    for (int j = c.enc_width; j < c.enc_padded; ++j) feed(1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// vMF mixture (src/render/guided/distribution.h, src/backend/src/util/vmf.h).
+template <int K>
+struct Mixture {
+    float lambda[K], kappa[K], lognorm[K], weight[K];
+    V3 mu[K];
+    float total;
+};
+
+// log of the stable normaliser kappa / (2 pi (-expm1(-2 kappa))), 1/4pi at 0.
+__device__ __forceinline__ float stable_lognorm(float k) {
+    if (!(k > 1e-30f)) return fm::log(kInv4Pi);
+    return fm::log(k) - fm::log(kTwoPi) - fm::log(-fm::expm1(-2.0f * k));
+}
+// Reference normaliser (vmf.h:22-27 and distribution.h:40-42) with its 1/4pi threshold.
+__device__ __forceinline__ float ref_norm(float k, float thresh) {
+    return k < thresh ? kInv4Pi : k * kInv2Pi / (1.0f - fm::exp(-2.0f * k));
+}
+
+// network_to_params (train.h:199-210) + constructor (distribution.h:85-98).
+// x points at the first raw output of the query; `stride` is the distance
+// between consecutive outputs. Mapping "softmax" reads (theta, phi, log
+// kappa, logit) per lobe.
+template <int K, class Raw>
+__device__ __forceinline__ void build_mixture(const DevCfg& c, Raw&& x, Mixture<K>& m) {
+    m.total = 0.0f;
+    float theta[K], phi[K];
+    if (c.mapping == MAP_REFERENCE) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            m.lambda[i] = fm::exp(clampv(x(4 * i + 0), -10.0f, 15.0f));
+            m.kappa[i] = fm::exp(clampv(x(4 * i + 1), -10.0f, 15.0f));
+            theta[i] = kPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 2))));
+            phi[i] = kTwoPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 3))));
+            m.total += m.lambda[i];
+        }
+    } else {
+        float mx = x(3);
+#pragma unroll
+        for (int i = 1; i < K; ++i) mx = smax(mx, x(4 * i + 3));
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            theta[i] = kPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 0))));
+            phi[i] = kTwoPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 1))));
+            m.kappa[i] = clampv(fm::exp(x(4 * i + 2)), 1e-2f, 1e3f);
+            m.lambda[i] = fm::exp(x(4 * i + 3) - mx);
+            m.total += m.lambda[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        m.weight[i] = m.lambda[i] / m.total;
+        m.mu[i] = sph_to_cart(theta[i], phi[i]);
+        m.lognorm[i] = stable_lognorm(m.kappa[i]);
+    }
+}
+
+// applyCosineLobe (distribution.h:184-195) with VMFKernel::product (:39-64).
+// Reference numerics take each new mean back through spherical coordinates
+// like the reference; stable numerics keep the normalised cartesian mean.
+template <int K>
+__device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>& m, bool want_angles,
+                                             float (&theta_out)[K], float (&phi_out)[K]) {
+    const bool ref = c.numerics == NUM_REFERENCE;
+    float nt, np;
+    cart_to_sph(nrm, &nt, &np);
+    const float kb = c.cos_kappa;
+    const V3 mu_b = sph_to_cart(nt, np);
+    const float norm_b = ref ? ref_norm(kb, 1e-3f) : fm::exp(stable_lognorm(kb));
+    m.total = 0.0f;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const V3 mu_a = m.mu[i];
+        const float ka = m.kappa[i];
+        V3 nm = add(mul(mu_a, ka), mul(mu_b, kb));
+        float nk = length(nm);
+        if (nk >= 1e-3f) {
+            nm = divs(nm, nk);
+        } else {
+            nk = 0.0f;
+            nm = mu_a;
+        }
+        V3 mu_new = nm;
+        if (ref || want_angles) {
+            float th, ph;
+            cart_to_sph(nm, &th, &ph);
+            if (ref) mu_new = sph_to_cart(th, ph);
+            theta_out[i] = th;
+            phi_out[i] = ph;
+        }
+        const float e = fm::exp(ka * (dot(nm, mu_a) - 1.0f) + kb * (dot(nm, mu_b) - 1.0f));
+        const float na = ref ? ref_norm(ka, 1e-3f) : fm::exp(stable_lognorm(ka));
+        const float nn = ref ? ref_norm(nk, 1e-3f) : fm::exp(stable_lognorm(nk));
+        const float s = na * norm_b / nn * e;
+        m.lambda[i] = s * m.lambda[i] * 1.0f;
+        m.kappa[i] = nk;
+        m.mu[i] = mu_new;
+        m.lognorm[i] = stable_lognorm(nk);
+        m.total += m.lambda[i];
+    }
+    m.total = smax(m.total, kEps);
+#pragma unroll
+    for (int i = 0; i < K; ++i) m.weight[i] = m.lambda[i] / m.total;
+}
+
+// One lobe's pdf at unit direction w (vmf.h:22-27, or the log-space form).
+template <int K>
+__device__ __forceinline__ float lobe_pdf(const DevCfg& c, const Mixture<K>& m, int i, V3 w) {
+    const float cth = dot(w, m.mu[i]);
+    const float d = smin(0.0f, cth - 1.0f);
+    if (c.numerics == NUM_REFERENCE) {
+        const float k = m.kappa[i];
+        if (k < kEps) return kInv4Pi;
+        return fm::exp(k * d) * k / (kTwoPi * (1.0f - fm::exp(-2.0f * k)));
+    }
+    return fm::exp(fm::fma_(m.kappa[i], d, m.lognorm[i]));
+}
+
+// distribution.h:113-119
+template <int K>
+__device__ __forceinline__ float mixture_pdf(const DevCfg& c, const Mixture<K>& m, V3 w) {
+    float p = 0.0f;
+#pragma unroll
+    for (int i = 0; i < K; ++i) p = fm::fma_(m.weight[i], lobe_pdf(c, m, i, w), p);
+    return p;
+}
+
+// distribution.h:132-140 (lobe choice by running subtraction, fallback lobe 0)
+// and vmf.h:48-71 (inverse-CDF sample about +z rotated into Frame(mu)).
+template <int K>
+__device__ __forceinline__ V3 mixture_sample(const DevCfg& c, const Mixture<K>& m, float u0, float u1, float u2,
+                                             uint32_t* lobe) {
+    int sel = -1;
+    float u = u0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        if (sel < 0) {
+            if (u < m.weight[i]) sel = i;
+            else u -= m.weight[i];
+        }
+    }
+    if (sel < 0) sel = 0;
+    *lobe = (uint32_t)sel;
+    // Bitwise selects (not `if (i == sel)`, which the compiler folds back into
+    // an indexed load that forces the lobe arrays out of registers).
+    uint32_t kb = fm::float_to_bits(m.kappa[0]), xb = fm::float_to_bits(m.mu[0].x),
+             yb = fm::float_to_bits(m.mu[0].y), zb = fm::float_to_bits(m.mu[0].z);
+#pragma unroll
+    for (int i = 1; i < K; ++i) {
+        const uint32_t mask = 0u - (uint32_t)(i == sel);
+        kb = (kb & ~mask) | (fm::float_to_bits(m.kappa[i]) & mask);
+        xb = (xb & ~mask) | (fm::float_to_bits(m.mu[i].x) & mask);
+        yb = (yb & ~mask) | (fm::float_to_bits(m.mu[i].y) & mask);
+        zb = (zb & ~mask) | (fm::float_to_bits(m.mu[i].z) & mask);
+    }
+    const float kap = fm::bits_to_float(kb);
+    const V3 mu = mk(fm::bits_to_float(xb), fm::bits_to_float(yb), fm::bits_to_float(zb));
+    const bool ref = c.numerics == NUM_REFERENCE;
+    V3 loc;
+    if (ref ? kap < kEps : !(kap > 1e-30f)) {
+        loc = uniform_sample_sphere(u1, u2);
+    } else {
+        float ct;
+        if (ref) ct = 1.0f + fm::log1p(-u1 + fm::exp(-2.0f * kap) * u1) / kap;
+        else ct = 1.0f + fm::log1p(u1 * fm::expm1(-2.0f * kap)) / kap;
+        const float st = fm::sqrt_(smax(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        fm::sincos(kTwoPi * u2, &sp, &cp);
+        loc = {cp * st, sp * st, ct};
+    }
+    return frame_to_world(mu, loc);
+}
+
+// Derivative of 1/k - coth(k): reference polynomial below 1 (distribution.h:
+// 162-168) or the exact value (series below 0.5).
+__device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
+    if (ref) {
+        if (k < 1.0f) return 0.000926f + -0.344883f * k + 0.030147f * (k * k);
+        return 1.0f / k - (1.0f + fm::exp(-2.0f * k)) / (1.0f - fm::exp(-2.0f * k));
+    }
+    if (k < 0.5f) {
+        const float k2 = k * k;
+        return -k * (1.0f / 3.0f - k2 * (1.0f / 45.0f - k2 * (2.0f / 945.0f - k2 * (1.0f / 4725.0f))));
+    }
+    const float e = fm::exp(-2.0f * k);
+    return 1.0f / k - (1.0f + e) / (1.0f - e);
+}
+
+// KL loss -f/q log(p + eps) and its adjoint with respect to the raw outputs
+// (train.cu:69-124 with gradients_probability, distribution.h:143-181).
+// dp/dlambda_s is evaluated as (v_s - p) / Lambda, the closed form of the
+// reference's O(K^2) expression. `ls_over_n` is loss_scale / batch size.
+template <int K, class Raw, class Out>
+__device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
+                                                 float b, float wipdf, float ls_over_n, Out&& out) {
+    const bool ref = c.numerics == NUM_REFERENCE;
+    Mixture<K> m;
+    build_mixture<K>(c, x, m);
+    const V3 wi = sph_to_cart(th_t, ph_t);
+    const float Li = (r + g + b) / 3.0f;
+    const float wiPdf = wipdf + kEps;
+    float vmf[K];
+    float p = 0.0f;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        vmf[s] = lobe_pdf(c, m, s, wi);
+        p = fm::fma_(m.weight[s], vmf[s], p);
+    }
+    const float guide = p + kEps;
+    const float prefix = -Li / wiPdf / guide * ls_over_n;
+    const float loss = -Li / wiPdf * fm::log(guide);
+    const float clampv_hi = c.grad_clamp;
+    auto finish = [&](int j, float v) {
+        if (!fm::isfinite_(v)) v = 0.0f;
+        if (clampv_hi > 0.0f) v = smin(clampv_hi, v);
+        out(j, v);
+    };
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const float kap = m.kappa[s];
+        const V3 mu = m.mu[s];
+        // mu = (sin t cos p, sin t sin p, cos t): recover the derivative
+        // directions from mu and sin t (no second trigonometric evaluation).
+        const float st = fm::sqrt_(smax(0.0f, mu.x * mu.x + mu.y * mu.y));
+        const float wv = m.weight[s] * vmf[s];
+        const float dl = (vmf[s] - p) / m.total;
+        const float dk = wv * (dot(wi, mu) + inv_k_minus_coth(kap, ref));
+        // d mu / d theta = (cos t cos p, cos t sin p, -sin t); d mu / d phi = sin t (-sin p, cos p, 0).
+        float cp = 1.0f, sp = 0.0f;
+        if (st > 0.0f) { cp = mu.x / st; sp = mu.y / st; }
+        const float dth = wv * kap * (mu.z * (wi.x * cp + wi.y * sp) - wi.z * st);
+        const float dph = wv * kap * (wi.y * cp - wi.x * sp) * st;
+        if (c.mapping == MAP_REFERENCE) {
+            const float x0 = x(4 * s + 0), x1 = x(4 * s + 1), x2 = x(4 * s + 2), x3 = x(4 * s + 3);
+            const float e0 = fm::exp(clampv(x0, -10.0f, 15.0f));
+            const float e1 = fm::exp(clampv(x1, -10.0f, 15.0f));
+            const float d0 = (!ref && (x0 < -10.0f || x0 > 15.0f)) ? 0.0f : e0;
+            const float d1 = (!ref && (x1 < -10.0f || x1 > 15.0f)) ? 0.0f : e1;
+            const float s2 = 1.0f / (1.0f + fm::exp(-x2));
+            const float s3 = 1.0f / (1.0f + fm::exp(-x3));
+            finish(4 * s + 0, prefix * dl * d0);
+            finish(4 * s + 1, prefix * dk * d1);
+            finish(4 * s + 2, prefix * dth * kPi * (s2 * (1.0f - s2)));
+            finish(4 * s + 3, prefix * dph * kTwoPi * (s3 * (1.0f - s3)));
+        } else {
+            const float x0 = x(4 * s + 0), x1 = x(4 * s + 1), x2 = x(4 * s + 2);
+            const float s0 = 1.0f / (1.0f + fm::exp(-x0));
+            const float s1 = 1.0f / (1.0f + fm::exp(-x1));
+            const float ek = fm::exp(x2);
+            const float dkx = (ek > 1e-2f && ek < 1e3f) ? ek : 0.0f;
+            finish(4 * s + 0, prefix * dth * kPi * (s0 * (1.0f - s0)));
+            finish(4 * s + 1, prefix * dph * kTwoPi * (s1 * (1.0f - s1)));
+            finish(4 * s + 2, prefix * dk * dkx);
+            // lambda_k = exp(logit_k - max): d lambda_k / d logit_s = delta_ks lambda_s.
+            finish(4 * s + 3, prefix * dl * m.lambda[s]);
+        }
+    }
+    for (int j = 4 * K; j < c.n_out_pad; ++j) out(j, 0.0f);
+    return loss;
+}
+
+}  // namespace dev
+}  // namespace npmg

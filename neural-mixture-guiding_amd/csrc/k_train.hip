@@ -1,0 +1,438 @@
+// k_train.hip -- FP32 training kernels: forward with stored activations and
+// fused mixture adjoint, MLP/grid backward, weight-gradient GEMM with a
+// deterministic split-K reduction, Adam + EMA, weight transposition and the
+// loss mean.
+#include "device.h"
+#include "kernels.h"
+
+namespace npmg {
+namespace {
+
+using namespace dev;
+
+template <int W>
+constexpr int block_size() { return W == 64 ? 256 : 128; }
+
+__host__ __device__ __forceinline__ uint32_t n_mlp_of(const DevCfg& c) {
+    const int L = c.n_layers - 1;
+    return c.layer_off[L] + (uint32_t)(c.layer_in[L] * c.layer_out[L]);
+}
+
+template <int N>
+__device__ __forceinline__ void axpy_row(const float* __restrict__ w, float v, float* acc) {
+    const float4* w4 = reinterpret_cast<const float4*>(w);
+#pragma unroll
+    for (int o4 = 0; o4 < N / 4; ++o4) {
+        const float4 ww = w4[o4];
+        acc[4 * o4 + 0] = fm::fma_(ww.x, v, acc[4 * o4 + 0]);
+        acc[4 * o4 + 1] = fm::fma_(ww.y, v, acc[4 * o4 + 1]);
+        acc[4 * o4 + 2] = fm::fma_(ww.z, v, acc[4 * o4 + 2]);
+        acc[4 * o4 + 3] = fm::fma_(ww.w, v, acc[4 * o4 + 3]);
+    }
+}
+
+// Forward for training: encoded features and post-ReLU activations are kept
+// in HBM for the weight gradients; the loss and dL/draw of the mixture-KL
+// adjoint are computed in the same thread (train.cu:69-124).
+template <int W, int K, bool LDS_W>
+__global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, const float* __restrict__ wt,
+                                                               const float* __restrict__ table,
+                                                               const float* __restrict__ in,
+                                                               const float* __restrict__ tgt_dir,
+                                                               const float* __restrict__ tgt_rgb,
+                                                               const float* __restrict__ tgt_pdf, float ls_over_n,
+                                                               const TrainWS ws, uint32_t n) {
+    constexpr int BS = block_size<W>();
+    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    extern __shared__ float smem[];
+    const uint32_t n_mlp = n_mlp_of(c);
+    const float* sw = wt;
+    float* act_base = smem;
+    if (LDS_W) {
+        for (uint32_t i = threadIdx.x; i < n_mlp; i += BS) smem[i] = wt[i];
+        __syncthreads();
+        sw = smem;
+        act_base = smem + ((n_mlp + 3u) & ~3u);
+    }
+    float* act = act_base + threadIdx.x;
+    for (uint32_t tile = blockIdx.x; (size_t)tile * BS < n; tile += gridDim.x) {
+        const uint32_t q = tile * BS + threadIdx.x;
+        if (q >= n) continue;
+        float qin[16];
+#pragma unroll
+        for (int d = 0; d < 16; ++d) qin[d] = d < c.n_in ? in[(size_t)d * n + q] : 0.0f;
+        float acc[W];
+#pragma unroll
+        for (int o = 0; o < W; ++o) acc[o] = 0.0f;
+        int j = 0;
+        encode_query(c, table, qin, [&](float v) {
+            ws.enc[(size_t)j * n + q] = v;
+            axpy_row<W>(sw + (size_t)j * W, v, acc);
+            ++j;
+        });
+        float out[NOUT];
+        for (int l = 1; l < c.n_layers; ++l) {
+#pragma unroll
+            for (int o = 0; o < W; ++o) {
+                const float a = acc[o] > 0.0f ? acc[o] : 0.0f;
+                act[o * BS] = a;
+                ws.acts[((size_t)(l - 1) * W + o) * n + q] = a;
+            }
+            const float* wl = sw + c.layer_off[l];
+            if (l < c.n_layers - 1) {
+#pragma unroll
+                for (int o = 0; o < W; ++o) acc[o] = 0.0f;
+                for (int i = 0; i < W; ++i) axpy_row<W>(wl + (size_t)i * W, act[i * BS], acc);
+            } else {
+#pragma unroll
+                for (int o = 0; o < NOUT; ++o) out[o] = 0.0f;
+                for (int i = 0; i < W; ++i) axpy_row<NOUT>(wl + (size_t)i * NOUT, act[i * BS], out);
+            }
+        }
+        if (tgt_dir) {
+            const float loss = mixture_adjoint<K>(
+                c, [&](int jj) { return out[jj]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
+                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls_over_n,
+                [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
+            ws.loss[q] = loss;
+        }
+    }
+}
+
+// Standalone adjoint from raw outputs (harness split pass).
+template <int K>
+__global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __restrict__ raw,
+                                                 const float* __restrict__ tgt_dir, const float* __restrict__ tgt_rgb,
+                                                 const float* __restrict__ tgt_pdf, float ls_over_n,
+                                                 float* __restrict__ dout, float* __restrict__ loss, uint32_t n) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n) return;
+    loss[q] = mixture_adjoint<K>(
+        c, [&](int j) { return raw[(size_t)j * n + q]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
+        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls_over_n,
+        [&](int j, float v) { dout[(size_t)j * n + q] = v; });
+}
+
+// Backward: g_{l-1} = (W_l^T g_l) masked by ReLU, from the output layer down;
+// then dL/denc = W_0^T g_0 and the grid scatter (tiny-cuda-nn
+// kernel_grid_backward, grid.h:386-509, atomic mode). w: row-major [out][in]
+// weights of every layer (the parameter vector), staged in LDS when they fit.
+template <int W, int NOUT, bool LDS_W>
+__global__ void __launch_bounds__(block_size<W>()) k_train_bwd(const DevCfg c, const float* __restrict__ w,
+                                                               const float* __restrict__ in, const TrainWS ws,
+                                                               float* __restrict__ denc,
+                                                               float* __restrict__ grid_grad, uint32_t n) {
+    constexpr int BS = block_size<W>();
+    extern __shared__ float smem[];
+    const uint32_t n_mlp = n_mlp_of(c);
+    const float* sw = w;
+    float* g_base = smem;
+    if (LDS_W) {
+        for (uint32_t i = threadIdx.x; i < n_mlp; i += BS) smem[i] = w[i];
+        __syncthreads();
+        sw = smem;
+        g_base = smem + ((n_mlp + 3u) & ~3u);
+    }
+    float* gcol = g_base + threadIdx.x;  // [max(W, NOUT)][BS]
+    const int H = c.n_layers - 1;
+    for (uint32_t tile = blockIdx.x; (size_t)tile * BS < n; tile += gridDim.x) {
+        const uint32_t q = tile * BS + threadIdx.x;
+        if (q >= n) continue;
+        for (int o = 0; o < NOUT; ++o) gcol[o * BS] = ws.dout[(size_t)o * n + q];
+        int ow = NOUT;
+        for (int l = H; l >= 1; --l) {
+            float acc[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) acc[i] = 0.0f;
+            const float* wl = sw + c.layer_off[l];
+            for (int o = 0; o < ow; ++o) axpy_row<W>(wl + (size_t)o * W, gcol[o * BS], acc);
+            const float* a = ws.acts + (size_t)(l - 1) * W * n + q;
+            float* gout = ws.g + (size_t)(l - 1) * W * n + q;
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const float v = a[(size_t)i * n] > 0.0f ? acc[i] : 0.0f;
+                gout[(size_t)i * n] = v;
+                gcol[i * BS] = v;
+            }
+            ow = W;
+        }
+        // Layer 0: dL/denc for the grid features (and every feature when denc is requested).
+        const int nin = c.layer_in[0];
+        const int ge = c.grid_enc;
+        const int gbeg = ge >= 0 ? c.enc_offset[ge] : 0;
+        const int gend = ge >= 0 ? gbeg + c.n_levels * c.F : 0;
+        const int jb = denc ? 0 : gbeg, je = denc ? nin : gend;
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        if (ge >= 0) {
+            const int b = c.enc_begin[ge];
+            px = in[(size_t)b * n + q];
+            py = in[(size_t)(b + 1) * n + q];
+            pz = in[(size_t)(b + 2) * n + q];
+        }
+        int cached_l = -1, nc = 0;
+        uint32_t idx[8];
+        float wk[8];
+        for (int j0 = jb & ~15; j0 < je; j0 += 16) {
+            float acc[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+            // W_0 is [W][nin] row-major: row o, columns j0..j0+15 (nin is a multiple of 16).
+            for (int o = 0; o < W; ++o) axpy_row<16>(sw + (size_t)o * nin + j0, gcol[o * BS], acc);
+            for (int i = 0; i < 16; ++i) {
+                const int jj = j0 + i;
+                if (jj < jb || jj >= je) continue;
+                if (denc) denc[(size_t)jj * n + q] = acc[i];
+                if (jj >= gbeg && jj < gend && grid_grad) {
+                    const int l = (jj - gbeg) / c.F, f = (jj - gbeg) % c.F;
+                    if (l != cached_l) {
+                        nc = level_corners(c, l, px, py, pz, idx, wk);
+                        cached_l = l;
+                    }
+                    float* gl = grid_grad + (size_t)c.offsets[l] * c.F + f;
+                    for (int k = 0; k < nc; ++k) atomicAdd(gl + (size_t)idx[k] * c.F, wk[k] * acc[i]);
+                }
+            }
+        }
+    }
+}
+
+// Weight gradients: one block per (64 x 64 tile of one layer's dW, split of
+// the batch). Partial sums go to partials[split][param]; k_reduce_splits adds
+// the splits in order, so the result does not depend on scheduling.
+struct WgradTiles {
+    int n;
+    int layer[64], o0[64], i0[64];
+};
+
+__global__ void __launch_bounds__(256) k_wgrad(const DevCfg c, const WgradTiles tiles, const TrainWS ws,
+                                               float* __restrict__ partials, uint32_t n, uint32_t q_per_split) {
+    __shared__ float sg[32][65];
+    __shared__ float sx[32][65];
+    const int t = blockIdx.x;
+    const int l = tiles.layer[t], o0 = tiles.o0[t], i0 = tiles.i0[t];
+    const int H = c.n_layers - 1;
+    const int iw = c.layer_in[l], ow = c.layer_out[l];
+    const float* G = l == H ? ws.dout : ws.g + (size_t)l * c.width * n;        // [ow][n]
+    const float* X = l == 0 ? ws.enc : ws.acts + (size_t)(l - 1) * c.width * n;  // [iw][n]
+    const uint32_t qb = blockIdx.y * q_per_split;
+    const uint32_t qe = min(n, qb + q_per_split);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads, 4 x 4 outputs each
+    float acc[4][4] = {};
+    for (uint32_t q0 = qb; q0 < qe; q0 += 32) {
+        // Load 64 rows x 32 samples of G and X (coalesced along the batch).
+        for (int r = threadIdx.x; r < 64 * 32; r += 256) {
+            const int row = r >> 5, col = r & 31;
+            const uint32_t qq = q0 + col;
+            const bool okq = qq < qe;
+            sg[col][row] = (okq && o0 + row < ow) ? G[(size_t)(o0 + row) * n + qq] : 0.0f;
+            sx[col][row] = (okq && i0 + row < iw) ? X[(size_t)(i0 + row) * n + qq] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 32; ++k) {
+            float gv[4], xv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) { gv[a] = sg[k][ty * 4 + a]; xv[a] = sx[k][tx * 4 + a]; }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fm::fma_(gv[a], xv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    const uint32_t n_mlp = n_mlp_of(c);
+    float* P = partials + (size_t)blockIdx.y * n_mlp + c.layer_off[l];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int o = o0 + ty * 4 + a, i = i0 + tx * 4 + b;
+            if (o < ow && i < iw) P[(size_t)o * iw + i] = acc[a][b];
+        }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_splits(const float* __restrict__ partials, uint32_t n_splits,
+                                                       uint32_t len, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < n_splits; ++k) s += partials[(size_t)k * len + i];
+    out[i] = s;
+}
+
+// tiny-cuda-nn optimizers/adam.h:47-119 followed by the EMA of
+// optimizers/ema.h:62-76 (full-precision EMA).
+__global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restrict__ master,
+                                              const float* __restrict__ grad, float* __restrict__ m,
+                                              float* __restrict__ v, uint32_t* __restrict__ steps,
+                                              float* __restrict__ ema) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    float g = grad[i] / a.loss_scale;
+    const bool skip = i >= a.n_matrix && g == 0.0f;
+    float w = master[i];
+    if (!skip) {
+        if (i < a.n_matrix) g += a.l2_reg * w;
+        const float g2 = g * g;
+        const float m1 = a.beta1 * m[i] + (1 - a.beta1) * g;
+        const float v1 = a.beta2 * v[i] + (1 - a.beta2) * g2;
+        m[i] = m1;
+        v[i] = v1;
+        const uint32_t t = ++steps[i];
+        const float b2t = fm::exp((float)t * fm::log(a.beta2)), b1t = fm::exp((float)t * fm::log(a.beta1));
+        const float lr_t = a.lr * (fm::sqrt_(1 - b2t) / (1 - b1t));
+        const float eff = smin(smax(lr_t / (fm::sqrt_(v1) + a.eps), 0.0f), 3.402823466e38f);
+        w = w - eff * m1;
+        master[i] = w;
+    }
+    if (a.ema) ema[i] = (ema[i] * a.ema_decay * a.ema_debias_old + w * (1 - a.ema_decay)) * a.ema_debias_new;
+}
+
+__global__ void __launch_bounds__(256) k_transpose_mlp(const DevCfg c, const float* __restrict__ w,
+                                                       float* __restrict__ wt) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_mlp_of(c)) return;
+    int l = 0;
+    while (l + 1 < c.n_layers && i >= c.layer_off[l + 1]) ++l;
+    const uint32_t r = i - c.layer_off[l];
+    const uint32_t iw = (uint32_t)c.layer_in[l], ow = (uint32_t)c.layer_out[l];
+    const uint32_t o = r / iw, ii = r % iw;
+    wt[c.layer_off[l] + ii * ow + o] = w[i];
+}
+
+__global__ void __launch_bounds__(1024) k_mean(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
+    __shared__ float red[1024];
+    float s = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) s += x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = n ? red[0] / (float)n : 0.0f;
+}
+
+template <int W, int K, bool LDS_W>
+hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, const float* in, const float* td,
+                        const float* tr, const float* tp, float ls, const TrainWS& ws, uint32_t n, int n_cu,
+                        hipStream_t s) {
+    constexpr int BS = block_size<W>();
+    const uint32_t n_mlp = n_mlp_of(c);
+    const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)W * BS * sizeof(float);
+    hipError_t e = hipFuncSetAttribute((const void*)k_train_fwd<W, K, LDS_W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const uint32_t tiles = (n + BS - 1) / BS;
+    const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
+    hipLaunchKernelGGL((k_train_fwd<W, K, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, in, td, tr, tp, ls,
+                       ws, n);
+    return hipGetLastError();
+}
+
+template <int W, int NOUT, bool LDS_W>
+hipError_t launch_bwd_t(const DevCfg& c, const float* w, const float* in, const TrainWS& ws, float* denc,
+                        float* grid_grad, uint32_t n, int n_cu, hipStream_t s) {
+    constexpr int BS = block_size<W>();
+    constexpr int GR = W > NOUT ? W : NOUT;
+    const uint32_t n_mlp = n_mlp_of(c);
+    const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)GR * BS * sizeof(float);
+    hipError_t e = hipFuncSetAttribute((const void*)k_train_bwd<W, NOUT, LDS_W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const uint32_t tiles = (n + BS - 1) / BS;
+    const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
+    hipLaunchKernelGGL((k_train_bwd<W, NOUT, LDS_W>), dim3(grid), dim3(BS), lds, s, c, w, in, ws, denc, grid_grad, n);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool fp32_weights_fit_lds(const DevCfg& c);
+
+hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
+                                     const float* td, const float* tr, const float* tp, float ls, const TrainWS& ws,
+                                     uint32_t n, int n_cu, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const bool lds = fp32_weights_fit_lds(c);
+    if (c.width == 64 && c.K == 8) return lds ? launch_fwd_t<64, 8, true>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s)
+                                              : launch_fwd_t<64, 8, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+    if (c.width == 64 && c.K == 16) return lds ? launch_fwd_t<64, 16, true>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s)
+                                               : launch_fwd_t<64, 16, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+    if (c.width == 128 && c.K == 8) return launch_fwd_t<128, 8, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+    if (c.width == 128 && c.K == 16) return launch_fwd_t<128, 16, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const float* in, const TrainWS& ws, float* denc,
+                                      float* grid_grad, uint32_t n, int n_cu, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const bool lds = fp32_weights_fit_lds(c);
+    if (c.width == 64 && c.n_out_pad == 32) return lds ? launch_bwd_t<64, 32, true>(c, w, in, ws, denc, grid_grad, n, n_cu, s)
+                                                       : launch_bwd_t<64, 32, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
+    if (c.width == 64 && c.n_out_pad == 64) return lds ? launch_bwd_t<64, 64, true>(c, w, in, ws, denc, grid_grad, n, n_cu, s)
+                                                       : launch_bwd_t<64, 64, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
+    if (c.width == 128 && c.n_out_pad == 32) return launch_bwd_t<128, 32, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
+    if (c.width == 128 && c.n_out_pad == 64) return launch_bwd_t<128, 64, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
+    return hipErrorInvalidValue;
+}
+
+size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split) {
+    uint32_t splits = (n + 4095) / 4096;
+    if (splits > 64) splits = 64;
+    if (splits < 1) splits = 1;
+    *n_splits = splits;
+    *q_per_split = ((n + splits - 1) / splits + 31) / 32 * 32;
+    return (size_t)splits * n_mlp_of(c);
+}
+
+hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,
+                             uint32_t n, hipStream_t s) {
+    if (n == 0) return hipMemsetAsync(dw, 0, n_mlp_of(c) * sizeof(float), s);
+    WgradTiles t{};
+    for (int l = 0; l < c.n_layers; ++l)
+        for (int o0 = 0; o0 < c.layer_out[l]; o0 += 64)
+            for (int i0 = 0; i0 < c.layer_in[l]; i0 += 64) {
+                if (t.n >= 64) return hipErrorInvalidValue;
+                t.layer[t.n] = l; t.o0[t.n] = o0; t.i0[t.n] = i0; ++t.n;
+            }
+    uint32_t splits, qps;
+    const size_t need = wgrad_partials_needed(c, n, &splits, &qps);
+    if (need > partials_cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_wgrad, dim3(t.n, splits), dim3(256), 0, s, c, t, ws, partials, n, qps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t len = n_mlp_of(c);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, splits, len, dw);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, float* m, float* v, uint32_t* steps,
+                       float* ema, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a, master, grad, m, v, steps, ema);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipStream_t s) {
+    const uint32_t n = n_mlp_of(c);
+    hipLaunchKernelGGL(k_transpose_mlp, dim3((n + 255) / 256), dim3(256), 0, s, c, w, wt);
+    return hipGetLastError();
+}
+
+hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mean, dim3(1), dim3(1024), 0, s, x, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* td, const float* tr, const float* tp,
+                          float ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255) / 256);
+    if (c.K == 8) hipLaunchKernelGGL((k_adjoint<8>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    else if (c.K == 16) hipLaunchKernelGGL((k_adjoint<16>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace npmg

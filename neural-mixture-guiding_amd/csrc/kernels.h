@@ -1,0 +1,81 @@
+// kernels.h -- launchers of the HIP kernels (host-callable, return hipError_t).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "config.h"
+
+namespace npmg {
+
+// Per-query inputs and outputs of the fused query path (all SoA, device).
+struct QueryIO {
+    const float* in;         // [n_in][n]
+    const float* normal;     // [3][n] or null (cosine product off)
+    const float* u;          // [3][n] or null (no sampling)
+    const float* eval_dirs;  // [n_eval][3][n]
+    int n_eval;
+    float* out_dir;          // [3][n]
+    float* out_pdf;          // [n]
+    float* out_eval_pdf;     // [n_eval][n]
+    uint32_t* out_lobe;      // [n] or null
+    float* raw_out;          // [n_out_pad][n] or null
+    float* lobes_out;        // [K][5][n] or null (consumer kernel only)
+};
+
+// Training-side activations kept for the backward pass (SoA, device).
+struct TrainWS {
+    float* enc;   // [enc_padded][n]
+    float* acts;  // [n_hidden][width][n] post-ReLU
+    float* g;     // [n_hidden][width][n] dL/d(pre-activation) of each hidden layer
+    float* dout;  // [n_out_pad][n]
+    float* loss;  // [n]
+};
+
+// FP32 fused query: encode -> MLP (FP32 VALU) -> mixture -> sample/pdf.
+// wt: MLP weights transposed per layer ([in][out]); table: grid parameters.
+hipError_t launch_query_fp32(const DevCfg& c, const float* wt, const float* table, const QueryIO& io, uint32_t n,
+                             int n_cu, hipStream_t s);
+// Consumer kernel: raw outputs -> mixture -> sample/pdf.
+hipError_t launch_mixture(const DevCfg& c, const float* raw, const QueryIO& io, uint32_t n, hipStream_t s);
+hipError_t launch_encode(const DevCfg& c, const float* table, const float* in, float* enc, uint32_t n, hipStream_t s);
+hipError_t launch_grid_corners(const DevCfg& c, const float* in, uint32_t* idx, float* w, uint32_t n, hipStream_t s);
+
+// Training (FP32): forward storing activations (+ adjoint when tgt != null).
+hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
+                                     const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
+                                     float ls_over_n, const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s);
+// Backward through the MLP and the grid: g of each hidden layer into ws.g,
+// dL/denc into denc (optional), grid gradient scattered into grid_grad.
+hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const float* in, const TrainWS& ws,
+                                      float* denc, float* grid_grad, uint32_t n, int n_cu, hipStream_t s);
+// Weight gradients dW_l = sum_q g_l x_l^T, deterministic split-K reduction.
+hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,
+                             uint32_t n, hipStream_t s);
+// Adam (+EMA) over all parameters.
+struct AdamArgs {
+    size_t n, n_matrix;
+    float loss_scale, lr, beta1, beta2, eps, l2_reg;
+    bool ema;
+    float ema_decay, ema_debias_old, ema_debias_new;
+};
+hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, float* m, float* v, uint32_t* steps,
+                       float* ema, hipStream_t s);
+// Transposes the MLP weights of every layer ([out][in] -> [in][out]).
+hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipStream_t s);
+// Mean of n values into out[0] (deterministic tree order).
+hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s);
+hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                          const float* tgt_pdf, float ls_over_n, float* dout, float* loss, uint32_t n, hipStream_t s);
+
+hipError_t launch_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,
+                                float* eval_dirs, hipStream_t s);
+struct FieldParams {
+    float mu0[3][3], axis[3][3], d[3][3], kappa[3], a[3];
+};
+FieldParams make_field(uint64_t seed);
+hipError_t launch_synth_train(uint64_t seed, uint64_t first, uint32_t n, const FieldParams& f, float* in,
+                              float* tgt_dir, float* tgt_rgb, float* tgt_pdf, hipStream_t s);
+hipError_t launch_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, hipStream_t s);
+
+}  // namespace npmg

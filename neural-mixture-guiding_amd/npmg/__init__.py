@@ -1,0 +1,376 @@
+"""Python harness of the MI355X neural parametric mixture guiding library.
+
+The product is the C ABI in ``include/npmg.h`` implemented by ``libnpmg.so``
+(hand-written HIP for gfx950). This module binds it with ctypes for tests,
+benchmarks and autograd wrapping; PyTorch is used only for device memory and
+streams. The classes mirror the interface the reference's guided integrator
+uses (``src/render/guided/integrator.cpp:555-702``): a network with
+``inference``/``forward``/``backward``, a trainer ``optimizer_step`` and the
+mixture consumers' ``sample``/``pdf``.
+
+There is no CPU fallback: every call goes to the HIP library, and importing a
+missing or unbuilt library raises ``ImportError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnpmg.so")
+HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "npmg.h"))
+
+NPMG_OK, NPMG_EINVAL, NPMG_ENOMEM, NPMG_EHIP, NPMG_ESTATE = range(5)
+NPMG_COSINE_AWARE = 1
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_int = ctypes.c_int
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); every function declared in include/npmg.h.
+SIGNATURES = {
+    "npmg_create": (_int, [ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
+    "npmg_destroy": (None, [_vp]),
+    "npmg_last_error": (ctypes.c_char_p, [_vp]),
+    "npmg_n_params": (_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "npmg_widths": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "npmg_grid_layout": (_int, [_vp, ctypes.POINTER(_u32), _vp, _vp, _vp]),
+    "npmg_set_params": (_int, [_vp, _vp, _sz, _int]),
+    "npmg_get_params": (_int, [_vp, _vp, _sz, _int, _int]),
+    "npmg_infer": (_int, [_vp, _vp, _vp, _u32, _vp]),
+    "npmg_sample_pdf": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
+    "npmg_mixture_sample_pdf": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
+    "npmg_train_step": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
+    "npmg_encode": (_int, [_vp, _vp, _vp, _u32, _int, _vp]),
+    "npmg_grid_corners": (_int, [_vp, _vp, _vp, _vp, _u32, _vp]),
+    "npmg_forward": (_int, [_vp, _vp, _vp, _u32, _int, _vp]),
+    "npmg_mixture_adjoint": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _f32, _vp, _vp, _vp]),
+    "npmg_backward": (_int, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "npmg_optimizer_step": (_int, [_vp, _vp, _f32, _vp]),
+    "npmg_train_state": (_int, [_vp, ctypes.POINTER(_f32), ctypes.POINTER(_u32)]),
+    "npmg_synth_queries": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "npmg_synth_train": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "npmg_debug_math": (_int, [_int, _vp, _vp, _u32, _vp, _vp]),
+}
+
+
+class NpmgError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"npmg status {status}: {message}")
+        self.status = status
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if not t.is_cuda:
+        raise ValueError("device tensor required")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor required")
+    return t.data_ptr()
+
+
+def _stream():
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+# ---------------------------------------------------------------------------
+# Configurations
+
+def reference_config() -> dict:
+    """The reference's hyper-parameters (common/configs/nn/base.json "nn")."""
+    return {
+        "loss": {"otype": "L2"},
+        "optimizer": {"otype": "Ema", "decay": 0.95,
+                      "nested": {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99,
+                                 "epsilon": 1e-15, "l2_reg": 1e-6, "adabound": False}},
+        "encoding": {"otype": "Composite", "nested": [
+            {"otype": "DenseGrid", "n_levels": 8, "n_features_per_level": 4, "base_resolution": 8,
+             "per_level_scale": 1.405, "interpolation": "Smoothstep", "n_dims_to_encode": 3},
+            {"n_dims_to_encode": 3, "otype": "SphericalHarmonics", "degree": 2},
+            {"n_dims_to_encode": 3, "otype": "OneBlob", "n_bins": 4}]},
+        "network": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+                    "n_neurons": 64, "n_hidden_layers": 3},
+        "mixture": {"otype": "vMF", "n_lobes": 8, "mapping": "reference", "numerics": "reference"},
+    }
+
+
+def benchmark_config(size: str = "full", precision: str = "fp32", n_lobes: int = 8, width: int = 64,
+                     hidden: Optional[int] = None) -> dict:
+    """Hash-grid benchmark configurations: "small" (8 levels x 2^15) or "full" (16 levels x 2^19)."""
+    small = size == "small"
+    grid = {"otype": "HashGrid", "n_dims_to_encode": 3, "n_levels": 8 if small else 16,
+            "n_features_per_level": 2, "log2_hashmap_size": 15 if small else 19,
+            "base_resolution": 16, "per_level_scale": 1.5, "interpolation": "Linear"}
+    return {
+        "encoding": {"otype": "Composite", "nested": [
+            grid, {"otype": "OneBlob", "n_dims_to_encode": 6, "n_bins": 16}]},
+        "network": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+                    "n_neurons": width, "n_hidden_layers": hidden if hidden is not None else (2 if small else 3)},
+        "optimizer": {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
+                      "l2_reg": 1e-6},
+        "mixture": {"otype": "vMF", "n_lobes": n_lobes, "mapping": "softmax", "numerics": "stable"},
+        "loss": {"scale": 128.0, "grad_clamp": 150.0},
+        "precision": precision,
+    }
+
+
+# ---------------------------------------------------------------------------
+
+class GuidingNetwork:
+    """Owns one npmg context: the guiding network, its optimizer and workspaces."""
+
+    def __init__(self, config: dict, seed: int = 7272):
+        self._lib = lib()
+        self.config = dict(config)
+        h = _vp()
+        st = self._lib.npmg_create(json.dumps(self.config).encode(), seed, ctypes.byref(h))
+        if st != NPMG_OK:
+            raise NpmgError(st, self._lib.npmg_last_error(None).decode())
+        self._h = h
+        a, b = _sz(), _sz()
+        self._check(self._lib.npmg_n_params(self._h, ctypes.byref(a), ctypes.byref(b)))
+        self.n_mlp, self.n_grid = a.value, b.value
+        self.n_params = self.n_mlp + self.n_grid
+        ew, po = _u32(), _u32()
+        self._check(self._lib.npmg_widths(self._h, ctypes.byref(ew), ctypes.byref(po)))
+        self.encoded_width, self.padded_output = ew.value, po.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.npmg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st != NPMG_OK:
+            raise NpmgError(st, self._lib.npmg_last_error(self._h).decode())
+
+    # -- layout ---------------------------------------------------------------
+    def grid_layout(self):
+        import numpy as np
+
+        n = _u32()
+        offs = np.zeros(33, np.uint32)
+        scales = np.zeros(32, np.float32)
+        res = np.zeros(32, np.uint32)
+        self._check(self._lib.npmg_grid_layout(self._h, ctypes.byref(n), offs.ctypes.data, scales.ctypes.data,
+                                               res.ctypes.data))
+        L = n.value
+        return offs[:L + 1].copy(), scales[:L].copy(), res[:L].copy()
+
+    # -- parameters -----------------------------------------------------------
+    def get_params(self, which: int = 0):
+        import torch
+
+        out = torch.empty(self.n_params, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_get_params(self._h, _ptr(out), self.n_params, 1, which))
+        return out
+
+    def set_params(self, params):
+        import torch
+
+        p = params.to(device="cuda", dtype=torch.float32).contiguous()
+        self._check(self._lib.npmg_set_params(self._h, _ptr(p), self.n_params, 1))
+
+    # -- inference ------------------------------------------------------------
+    def inference(self, inputs):
+        """Network::inference: in [9][n] -> raw outputs [n_out][n] (inference/EMA weights)."""
+        return self.forward(inputs, use_inference_params=True)
+
+    def forward(self, inputs, use_inference_params: bool = False):
+        import torch
+
+        n = inputs.shape[1]
+        out = torch.empty(self.padded_output, n, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_forward(self._h, _ptr(inputs), _ptr(out), n, int(use_inference_params),
+                                           _stream()))
+        return out
+
+    def encode(self, inputs, use_inference_params: bool = False):
+        import torch
+
+        n = inputs.shape[1]
+        out = torch.empty(self.encoded_width, n, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_encode(self._h, _ptr(inputs), _ptr(out), n, int(use_inference_params), _stream()))
+        return out
+
+    def grid_corners(self, inputs):
+        import torch
+
+        n = inputs.shape[1]
+        L = len(self.grid_layout()[1])
+        idx = torch.empty(L, 8, n, dtype=torch.int32, device="cuda")
+        w = torch.empty(L, 8, n, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_grid_corners(self._h, _ptr(inputs), _ptr(idx), _ptr(w), n, _stream()))
+        return idx, w
+
+    def sample_pdf(self, inputs, normal=None, u=None, eval_dirs=None, cosine_aware: bool = True,
+                   raw: bool = False):
+        """Fused encode -> MLP -> mixture -> sample + pdfs (infer + sample + pdf)."""
+        import torch
+
+        n = inputs.shape[1]
+        n_eval = 0 if eval_dirs is None else eval_dirs.shape[0]
+        dev = "cuda"
+        out = {}
+        if u is not None:
+            out["dir"] = torch.empty(3, n, dtype=torch.float32, device=dev)
+            out["pdf"] = torch.empty(n, dtype=torch.float32, device=dev)
+            out["lobe"] = torch.empty(n, dtype=torch.int32, device=dev)
+        if n_eval:
+            out["eval_pdf"] = torch.empty(n_eval, n, dtype=torch.float32, device=dev)
+        if raw:
+            out["raw"] = torch.empty(self.padded_output, n, dtype=torch.float32, device=dev)
+        flags = NPMG_COSINE_AWARE if (cosine_aware and normal is not None) else 0
+        self._check(self._lib.npmg_sample_pdf(
+            self._h, _ptr(inputs), _ptr(normal) if flags else None, _ptr(u), _ptr(eval_dirs), n_eval,
+            _ptr(out.get("dir")), _ptr(out.get("pdf")), _ptr(out.get("eval_pdf")), _ptr(out.get("lobe")),
+            _ptr(out.get("raw")), n, flags, _stream()))
+        return out
+
+    def mixture_sample_pdf(self, raw_out, normal=None, u=None, eval_dirs=None, cosine_aware: bool = True,
+                           lobes: bool = False):
+        """Consumer side: raw outputs -> mixture -> sample + pdfs (+ lobe parameters)."""
+        import torch
+
+        n = raw_out.shape[1]
+        n_eval = 0 if eval_dirs is None else eval_dirs.shape[0]
+        K = self.config.get("mixture", {}).get("n_lobes", 8)
+        dev = "cuda"
+        out = {}
+        if u is not None:
+            out["dir"] = torch.empty(3, n, dtype=torch.float32, device=dev)
+            out["pdf"] = torch.empty(n, dtype=torch.float32, device=dev)
+            out["lobe"] = torch.empty(n, dtype=torch.int32, device=dev)
+        if n_eval:
+            out["eval_pdf"] = torch.empty(n_eval, n, dtype=torch.float32, device=dev)
+        if lobes:
+            out["lobes"] = torch.empty(K * 5, n, dtype=torch.float32, device=dev)
+        flags = NPMG_COSINE_AWARE if (cosine_aware and normal is not None) else 0
+        self._check(self._lib.npmg_mixture_sample_pdf(
+            self._h, _ptr(raw_out), _ptr(normal) if flags else None, _ptr(u), _ptr(eval_dirs), n_eval,
+            _ptr(out.get("dir")), _ptr(out.get("pdf")), _ptr(out.get("eval_pdf")), _ptr(out.get("lobe")),
+            _ptr(out.get("lobes")), n, flags, _stream()))
+        return out
+
+    # -- training -------------------------------------------------------------
+    def mixture_adjoint(self, raw_out, tgt_dir, tgt_rgb, tgt_pdf, loss_scale: float = 1.0):
+        import torch
+
+        n = raw_out.shape[1]
+        dout = torch.empty(self.padded_output, n, dtype=torch.float32, device="cuda")
+        loss = torch.empty(n, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_mixture_adjoint(self._h, _ptr(raw_out), _ptr(tgt_dir), _ptr(tgt_rgb),
+                                                   _ptr(tgt_pdf), n, float(loss_scale), _ptr(dout), _ptr(loss),
+                                                   _stream()))
+        return dout, loss
+
+    def backward(self, inputs, dout, want_denc: bool = False):
+        import torch
+
+        n = inputs.shape[1]
+        dparams = torch.empty(self.n_params, dtype=torch.float32, device="cuda")
+        denc = torch.empty(self.encoded_width, n, dtype=torch.float32, device="cuda") if want_denc else None
+        self._check(self._lib.npmg_backward(self._h, _ptr(inputs), _ptr(dout.contiguous()), n, _ptr(dparams),
+                                            _ptr(denc), _stream()))
+        return dparams, denc
+
+    def optimizer_step(self, dparams, loss_scale: float):
+        """Trainer::optimizer_step (trainer.h:153-155)."""
+        self._check(self._lib.npmg_optimizer_step(self._h, _ptr(dparams), float(loss_scale), _stream()))
+
+    def train_step(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, loss_out=None):
+        """One full training step; returns the device scalar with the mean loss."""
+        import torch
+
+        n = inputs.shape[1]
+        if loss_out is None:
+            loss_out = torch.empty(1, dtype=torch.float32, device="cuda")
+        self._check(self._lib.npmg_train_step(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb), _ptr(tgt_pdf),
+                                               n, _ptr(loss_out), _stream()))
+        return loss_out
+
+    def train_state(self):
+        s, t = _f32(), _u32()
+        self._check(self._lib.npmg_train_state(self._h, ctypes.byref(s), ctypes.byref(t)))
+        return s.value, t.value
+
+
+# ---------------------------------------------------------------------------
+# Synthetic wavefront records
+
+def synth_queries(seed: int, n: int, first: int = 0):
+    import torch
+
+    d = "cuda"
+    t = dict(inputs=torch.empty(9, n, device=d), normal=torch.empty(3, n, device=d),
+             u=torch.empty(3, n, device=d), eval_dirs=torch.empty(4, 3, n, device=d))
+    st = lib().npmg_synth_queries(seed, first, n, _ptr(t["inputs"]), _ptr(t["normal"]), _ptr(t["u"]),
+                                  _ptr(t["eval_dirs"]), _stream())
+    if st != NPMG_OK:
+        raise NpmgError(st, "npmg_synth_queries failed")
+    return t
+
+
+def synth_train(seed: int, n: int, first: int = 0):
+    import torch
+
+    d = "cuda"
+    t = dict(inputs=torch.empty(9, n, device=d), tgt_dir=torch.empty(2, n, device=d),
+             tgt_rgb=torch.empty(3, n, device=d), tgt_pdf=torch.empty(n, device=d))
+    st = lib().npmg_synth_train(seed, first, n, _ptr(t["inputs"]), _ptr(t["tgt_dir"]), _ptr(t["tgt_rgb"]),
+                                _ptr(t["tgt_pdf"]), _stream())
+    if st != NPMG_OK:
+        raise NpmgError(st, "npmg_synth_train failed")
+    return t
+
+
+def debug_math(fn: int, x, y=None):
+    import torch
+
+    out = torch.empty_like(x)
+    st = lib().npmg_debug_math(fn, _ptr(x), _ptr(y), x.numel(), _ptr(out), _stream())
+    if st != NPMG_OK:
+        raise NpmgError(st, "npmg_debug_math failed")
+    return out
+
+
+from .autograd import NetworkFunction, network_apply  # noqa: E402,F401
