@@ -86,7 +86,22 @@ _vp = ctypes.c_void_p
 
 
 def _p(a):
+    """Pointer to an array that the caller keeps alive for the duration of the call."""
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class _Keep:
+    """Holds converted temporaries so that their memory outlives the C call."""
+
+    def __init__(self):
+        self.refs = []
+
+    def __call__(self, a, dtype=np.float32):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype)
+        self.refs.append(a)
+        return ctypes.c_void_p(a.ctypes.data)
 
 
 class Oracle:
@@ -125,30 +140,33 @@ class Oracle:
         return offs[:L + 1], scales[:L], res[:L]
 
     def grid_corners(self, cfg, inputs):
+        k = _Keep()
         n = inputs.shape[1]
         idx = np.zeros((cfg.n_levels, 8, n), np.uint32)
         w = np.zeros((cfg.n_levels, 8, n), np.float32)
-        self.lib.orc_grid_corners(ctypes.byref(cfg), _p(np.ascontiguousarray(inputs, np.float32)), n, _p(idx), _p(w))
+        self.lib.orc_grid_corners(ctypes.byref(cfg), k(inputs, np.float32), n, _p(idx), _p(w))
         return idx, w
 
     # -- forward ----------------------------------------------------------
     def encode(self, cfg, grid_params, inputs, dtype=np.float32):
+        k = _Keep()
         n = inputs.shape[1]
         ew, _ = self.widths(cfg)
         out = np.zeros((ew, n), dtype)
         fn = self.lib.orc_encode_f if dtype == np.float32 else self.lib.orc_encode_d
-        fn(ctypes.byref(cfg), _p(np.ascontiguousarray(grid_params, np.float32)),
-           _p(np.ascontiguousarray(inputs, np.float32)), n, _p(out))
+        fn(ctypes.byref(cfg), k(grid_params, np.float32),
+           k(inputs, np.float32), n, _p(out))
         return out
 
     def mlp_forward(self, cfg, mlp_params, enc, acts=False):
+        k = _Keep()
         dtype = enc.dtype
         n = enc.shape[1]
         _, po = self.widths(cfg)
         raw = np.zeros((po, n), dtype)
         a = np.zeros((cfg.n_hidden_layers, cfg.width, n), dtype) if acts else None
         fn = self.lib.orc_mlp_forward_f if dtype == np.float32 else self.lib.orc_mlp_forward_d
-        fn(ctypes.byref(cfg), _p(np.ascontiguousarray(mlp_params, np.float32)), _p(np.ascontiguousarray(enc)), n,
+        fn(ctypes.byref(cfg), k(mlp_params, np.float32), k(enc, enc.dtype), n,
            _p(raw), _p(a))
         return (raw, a) if acts else raw
 
@@ -174,6 +192,7 @@ class Oracle:
         return res
 
     def query(self, cfg, params, inputs, normal=None, u=None, eval_dirs=None, dtype=np.float32, n_threads=8):
+        k = _Keep()
         n = inputs.shape[1]
         n_eval = 0 if eval_dirs is None else eval_dirs.shape[0]
         _, po = self.widths(cfg)
@@ -184,7 +203,7 @@ class Oracle:
         raw = np.zeros((po, n), dtype)
         fn = self.lib.orc_query_f if dtype == np.float32 else self.lib.orc_query_d
         c32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
-        fn(ctypes.byref(cfg), _p(c32(params)), _p(c32(inputs)), _p(c32(normal)), _p(c32(u)), _p(c32(eval_dirs)),
+        fn(ctypes.byref(cfg), k(c32(params)), k(c32(inputs)), k(c32(normal)), k(c32(u)), k(c32(eval_dirs)),
            n_eval, n, _p(out_dir), _p(out_pdf), _p(out_eval), _p(lobe), _p(raw), n_threads)
         res = {"raw": raw}
         if u is not None:
@@ -195,34 +214,37 @@ class Oracle:
 
     # -- training ---------------------------------------------------------
     def adjoint(self, cfg, raw, tgt_dir, tgt_rgb, tgt_pdf, loss_scale=1.0, dtype=np.float32):
+        k = _Keep()
         n = raw.shape[1]
         _, po = self.widths(cfg)
         dout = np.zeros((po, n), dtype)
         loss = np.zeros(n, dtype)
         fn = self.lib.orc_adjoint_f if dtype == np.float32 else self.lib.orc_adjoint_d
         c32 = lambda a: np.ascontiguousarray(a, np.float32)
-        fn(ctypes.byref(cfg), _p(np.ascontiguousarray(raw, dtype)), _p(c32(tgt_dir)), _p(c32(tgt_rgb)),
-           _p(c32(tgt_pdf)), n, loss_scale, _p(dout), _p(loss))
+        fn(ctypes.byref(cfg), k(raw, dtype), k(c32(tgt_dir)), k(c32(tgt_rgb)),
+           k(c32(tgt_pdf)), n, loss_scale, _p(dout), _p(loss))
         return dout, loss
 
     def mlp_backward(self, cfg, mlp_params, enc, acts, dout):
+        k = _Keep()
         dtype = enc.dtype
         n = enc.shape[1]
         n_mlp, _ = self.n_params(cfg)
         dw = np.zeros(n_mlp, dtype)
         denc = np.zeros_like(enc)
         fn = self.lib.orc_mlp_backward_f if dtype == np.float32 else self.lib.orc_mlp_backward_d
-        fn(ctypes.byref(cfg), _p(np.ascontiguousarray(mlp_params, np.float32)), _p(np.ascontiguousarray(enc)),
-           _p(np.ascontiguousarray(acts)), _p(np.ascontiguousarray(dout, dtype)), n, _p(dw), _p(denc))
+        fn(ctypes.byref(cfg), k(mlp_params, np.float32), k(enc, enc.dtype),
+           k(acts, acts.dtype), k(dout, dtype), n, _p(dw), _p(denc))
         return dw, denc
 
     def grid_backward(self, cfg, inputs, denc):
+        k = _Keep()
         dtype = denc.dtype
         n = inputs.shape[1]
         _, n_grid = self.n_params(cfg)
         gg = np.zeros(n_grid, dtype)
         fn = self.lib.orc_grid_backward_f if dtype == np.float32 else self.lib.orc_grid_backward_d
-        fn(ctypes.byref(cfg), _p(np.ascontiguousarray(inputs, np.float32)), _p(np.ascontiguousarray(denc)), n, _p(gg))
+        fn(ctypes.byref(cfg), k(inputs, np.float32), k(denc, denc.dtype), n, _p(gg))
         return gg
 
     def adam_step(self, n_matrix, loss_scale, lr, beta1, beta2, eps, l2, master, grad, m, v, steps):
@@ -253,14 +275,16 @@ class Oracle:
         return a
 
     def field_eval(self, seed, pos, dirs):
+        k = _Keep()
         n = pos.shape[1]
         f = np.zeros(n, np.float64)
-        self.lib.orc_field_eval(seed, _p(np.ascontiguousarray(pos, np.float32)),
-                                _p(np.ascontiguousarray(dirs, np.float32)), n, _p(f))
+        self.lib.orc_field_eval(seed, k(pos, np.float32),
+                                k(dirs, np.float32), n, _p(f))
         return f
 
     def math(self, fn, x, y=None):
         x = np.ascontiguousarray(x, np.float32)
         out = np.zeros_like(x)
-        self.lib.orc_math(fn, _p(x), _p(None if y is None else np.ascontiguousarray(y, np.float32)), x.size, _p(out))
+        y = None if y is None else np.ascontiguousarray(y, np.float32)
+        self.lib.orc_math(fn, _p(x), _p(y), x.size, _p(out))
         return out

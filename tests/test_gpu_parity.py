@@ -175,16 +175,14 @@ def test_adjoint_vs_fp64(nets, oracle, name):
     import pyoracle
 
     net = nets(name)
-    cfg = dict(net.config, loss={"scale": 128.0, "grad_clamp": 0.0})
-    oc = pyoracle.make_cfg(cfg)
-    oc_clamped = pyoracle.make_cfg(net.config)
+    oc = pyoracle.make_cfg(net.config)
     n = 2048
     t = npmg.synth_train(3, n)
     rng = np.random.default_rng(2)
     raw = rng.normal(0, 1.0, size=(net.padded_output, n)).astype(np.float32)
     dout, loss = net.mixture_adjoint(torch.from_numpy(raw).cuda(), t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], 128.0)
     h = {k: _np(v) for k, v in t.items()}
-    gd, gl = oracle.adjoint(oc_clamped, raw, h["tgt_dir"], h["tgt_rgb"], h["tgt_pdf"], 128.0, dtype=np.float64)
+    gd, gl = oracle.adjoint(oc, raw, h["tgt_dir"], h["tgt_rgb"], h["tgt_pdf"], 128.0, dtype=np.float64)
     d = _np(dout)
     scale = np.abs(gd).max(axis=0, keepdims=True) + 1e-30
     assert np.max(np.abs(d - gd) / scale) < 1e-4
