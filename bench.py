@@ -27,6 +27,7 @@ N_QUERIES = 1 << 21
 N_TRAIN = 1 << 18
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 BYTES_PER_QUERY = 36 + 12 + 48 + 12 + 4 + 16 + 4  # in, u, 4 eval dirs; dir, pdf, 4 pdfs, lobe
 
 
@@ -79,7 +80,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -93,9 +94,6 @@ def main():
 
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
-    cfg = npmg.benchmark_config("full", precision=args.precision)
-    net = npmg.GuidingNetwork(cfg, seed=7272)
-    log(f"network: {net.n_mlp} MLP + {net.n_grid} grid parameters")
     q = npmg.synth_queries(2024, N_QUERIES)
     out = {
         "dir": torch.empty(3, N_QUERIES, device="cuda"), "pdf": torch.empty(N_QUERIES, device="cuda"),
@@ -104,16 +102,31 @@ def main():
     lib = npmg.lib()
     P = npmg._ptr
 
-    def query_step():
-        st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
-                                 P(out["dir"]), P(out["pdf"]), P(out["eval_pdf"]), P(out["lobe"]), None,
-                                 N_QUERIES, npmg.NPMG_COSINE_AWARE, stream.cuda_stream)
-        if st != 0:
-            raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
+    def measure_queries(precision):
+        net = npmg.GuidingNetwork(npmg.benchmark_config("full", precision=precision), seed=7272)
 
-    ms_q = time_region(query_step, args.steps, args.warmup, stream)
+        def query_step():
+            st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
+                                     P(out["dir"]), P(out["pdf"]), P(out["eval_pdf"]), P(out["lobe"]), None,
+                                     N_QUERIES, npmg.NPMG_COSINE_AWARE, stream.cuda_stream)
+            if st != 0:
+                raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
+
+        ms = time_region(query_step, args.steps, args.warmup, stream)
+        log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
+        return net, ms
+
+    per_prec = {}
+    for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
+        netp, ms = measure_queries(prec)
+        per_prec[prec] = ms
+        if prec != args.precision:
+            netp.close()
+        else:
+            net = netp
+    log(f"network: {net.n_mlp} MLP + {net.n_grid} grid parameters")
+    ms_q = per_prec[args.precision]
     qps = N_QUERIES / (ms_q * 1e-3)
-    log(f"infer+sample+pdf: {ms_q:.3f} ms per 2^21 queries ({qps / 1e6:.1f} M queries/s)")
 
     t = npmg.synth_train(77, N_TRAIN)
     loss = torch.empty(1, device="cuda")
@@ -129,6 +142,7 @@ def main():
     log(f"train step: {ms_t:.3f} ms per 2^18 samples ({sps / 1e6:.2f} M samples/s), loss {loss.item():.4f}")
 
     table_bytes = net.n_grid * (4 if args.precision == "fp32" else 2)
+    half = args.precision != "fp32"
     alg_bytes = BYTES_PER_QUERY * N_QUERIES + table_bytes
     achieved = alg_bytes / (ms_q * 1e-3) / 1e9
     flops = 2 * net.n_mlp * N_QUERIES
@@ -152,15 +166,19 @@ def main():
         "train_ms_per_step": ms_t,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_query_fp32 (fused encode+MLP+mixture+sample+pdf)",
+                     "kernel": ("k_query_mfma" if half else "k_query_fp32") +
+                               " (fused encode+MLP+mixture+sample+pdf)",
                      "algorithmic_bytes_per_launch": alg_bytes},
         "mlp_compute": {"flops_per_launch": flops, "achieved_tflops": flops / (ms_q * 1e-3) / 1e12,
-                        "peak_tflops": FP32_PEAK_TFLOPS},
+                        "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
+        "queries_per_s_by_precision": {k: N_QUERIES / (v * 1e-3) for k, v in per_prec.items()},
     }
+    if "fp32" in per_prec and half:
+        result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]
     if not args.no_cpu:
         params = net.get_params(1).cpu().numpy()
         q_host = {k: v.cpu().numpy() for k, v in q.items()}
-        result["cpu_baseline"] = cpu_baseline(cfg, params, q_host)
+        result["cpu_baseline"] = cpu_baseline(net.config, params, q_host)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.0f} queries/s")
     net.close()
     print(json.dumps(result), flush=True)

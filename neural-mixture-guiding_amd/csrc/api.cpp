@@ -26,6 +26,10 @@
 namespace npmg {
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const QueryIO& io,
+                             uint32_t n, int n_cu, hipStream_t s);
+hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
+hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
 }  // namespace npmg
 
 using namespace npmg;
@@ -134,6 +138,11 @@ struct npmg_ctx {
     float* grad = nullptr;
     float* wt_train = nullptr;  // transposed MLP weights
     float* wt_infer = nullptr;
+    // half-precision copies for the MFMA path: weight fragments and grid tables
+    void* frag_train = nullptr;
+    void* frag_infer = nullptr;
+    void* htab_train = nullptr;
+    void* htab_infer = nullptr;
     // training workspace
     uint32_t ws_cap = 0;
     TrainWS ws{};
@@ -147,18 +156,39 @@ struct npmg_ctx {
     ~npmg_ctx() {
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
-                        (void*)partials})
+                        (void*)partials, frag_train, frag_infer, htab_train, htab_infer})
             if (p) (void)hipFree(p);
     }
 
+    bool half() const { return cfg.precision == PREC_BF16 || cfg.precision == PREC_FP16; }
+
     void check_fused() const {
-        if (cfg.precision != PREC_FP32) throw std::invalid_argument("only precision fp32 is available in this build");
+        if (cfg.precision == PREC_FP8) throw std::invalid_argument("precision fp8 is not available in this build");
         if (cfg.d.K != 8 && cfg.d.K != 16) throw std::invalid_argument("the fused kernels support 8 or 16 lobes");
     }
 
     void refresh_layouts(hipStream_t s) {
         hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
         hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
+        if (half()) {
+            hip_check(launch_pack_frags(cfg.d, cfg.precision, master, frag_train, s), "pack");
+            hip_check(launch_pack_frags(cfg.d, cfg.precision, infer_params(), frag_infer, s), "pack");
+            hip_check(launch_to_half(cfg.precision, master + cfg.n_mlp, htab_train, cfg.n_grid, s), "table");
+            hip_check(launch_to_half(cfg.precision, infer_params() + cfg.n_mlp, htab_infer, cfg.n_grid, s), "table");
+        }
+    }
+
+    // Fused query with the training (0) or inference (1) parameters in the configured precision.
+    void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s) {
+        if (half()) {
+            hip_check(launch_query_mfma(cfg.d, cfg.precision, inference ? frag_infer : frag_train,
+                                        inference ? htab_infer : htab_train, io, n, n_cu, s),
+                      "query (mfma)");
+        } else {
+            const float* p = inference ? infer_params() : master;
+            hip_check(launch_query_fp32(cfg.d, inference ? wt_infer : wt_train, p + cfg.n_mlp, io, n, n_cu, s),
+                      "query (fp32)");
+        }
     }
 
     void ensure_ws(uint32_t n) {
@@ -282,6 +312,13 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         ctx->grad = dalloc<float>(c.n_params);
         ctx->wt_train = dalloc<float>(c.n_mlp);
         ctx->wt_infer = dalloc<float>(c.n_mlp);
+        if (ctx->half()) {
+            need(c.n_mlp % 512 == 0, "half precision needs layer sizes that are multiples of 32 x 16");
+            ctx->frag_train = dalloc<uint16_t>(c.n_mlp);
+            ctx->frag_infer = dalloc<uint16_t>(c.n_mlp);
+            ctx->htab_train = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
+            ctx->htab_infer = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
+        }
         hip_check(hipMemset(ctx->m, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->v, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->steps, 0, c.n_params * sizeof(uint32_t)), "memset");
@@ -375,9 +412,7 @@ npmg_status npmg_forward(npmg_ctx* ctx, const float* in, float* raw_out, uint32_
         QueryIO io{};
         io.in = in;
         io.raw_out = raw_out;
-        const float* wt = use_inference_params ? ctx->wt_infer : ctx->wt_train;
-        const float* p = use_inference_params ? ctx->infer_params() : ctx->master;
-        hip_check(launch_query_fp32(ctx->cfg.d, wt, p + ctx->cfg.n_mlp, io, n, ctx->n_cu, S(stream)), "forward");
+        ctx->query(io, n, use_inference_params != 0, S(stream));
     });
 }
 
@@ -403,9 +438,7 @@ npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal,
         io.out_eval_pdf = out_eval_pdf;
         io.out_lobe = out_lobe;
         io.raw_out = raw_out;
-        hip_check(launch_query_fp32(ctx->cfg.d, ctx->wt_infer, ctx->infer_params() + ctx->cfg.n_mlp, io, n, ctx->n_cu,
-                                    S(stream)),
-                  "query");
+        ctx->query(io, n, true, S(stream));
     });
 }
 

@@ -181,12 +181,14 @@ __device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
     return smax(0.0f, smin(1.0f, v));
 }
 
-// q_in[d] for a runtime d without indexing private memory (keeps q_in in VGPRs).
+// q_in[d] for a runtime d without indexing private memory (keeps q_in in
+// VGPRs). Integer masks rather than `d == k ? q_in[k] : v`, which the
+// compiler folds back into an indexed scratch load.
 __device__ __forceinline__ float pick(const float* q_in, int d) {
-    float v = q_in[0];
+    uint32_t v = 0u;
 #pragma unroll
-    for (int k = 1; k < 16; ++k) v = d == k ? q_in[k] : v;
-    return v;
+    for (int k = 0; k < 16; ++k) v |= fm::float_to_bits(q_in[k]) & (0u - (uint32_t)(d == k));
+    return fm::bits_to_float(v);
 }
 
 // Feeds every encoded feature of one query, in order, to `feed(value)`:
@@ -528,6 +530,140 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
     }
     for (int j = 4 * K; j < c.n_out_pad; ++j) out(j, 0.0f);
     return loss;
+}
+
+}  // namespace dev
+}  // namespace npmg
+
+namespace npmg {
+namespace dev {
+
+// ---------------------------------------------------------------------------
+// Chunked encoding: the 8 features [8 ch, 8 ch + 8) of one query, bit-identical
+// to the values encode_query produces (same expressions, same order), so the
+// MFMA kernels see exactly the FP32 path's features before rounding.
+
+// Sum of the three wrapped quartic CDFs at bin boundary b (oneblob.h:114-123).
+__device__ __forceinline__ float oneblob_cdf(float b, float x, float fnb) {
+    return quartic_cdf(b - x, fnb) + quartic_cdf(b - x - 1.0f, fnb) + quartic_cdf(b - x + 1.0f, fnb);
+}
+
+// Features of one grid level (F values) accumulated over the corners in order.
+template <class T, int F>
+__device__ __forceinline__ void grid_level_features(const DevCfg& c, const T* __restrict__ table, int l, float x0,
+                                                    float x1, float x2, float* out) {
+    uint32_t idx[8];
+    float w[8];
+    const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+    const T* tl = table + (size_t)c.offsets[l] * F;
+    if (nc == 1) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) out[f] = (float)tl[idx[0] * (uint32_t)F + f];
+        return;
+    }
+    T v[8][F];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[k][f] = tl[idx[k] * (uint32_t)F + f];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a = fm::fma_(w[k], (float)v[k][f], a);
+        out[f] = a;
+    }
+}
+
+// One feature, computed on its own (irregular chunks only).
+template <class T>
+__device__ __forceinline__ float single_feature(const DevCfg& c, const T* __restrict__ table, const float* q_in, int f) {
+    if (f >= c.enc_width) return 1.0f;
+    int e = 0;
+    for (int i = 1; i < c.n_enc; ++i)
+        if (f >= c.enc_offset[i]) e = i;
+    const int r = f - c.enc_offset[e];
+    const int b = c.enc_begin[e];
+    const int type = c.enc_type[e];
+    if (type == ENC_GRID) {
+        const int l = r / c.F, ff = r % c.F;
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, pick(q_in, b), pick(q_in, b + 1), pick(q_in, b + 2), idx, w);
+        const T* tl = table + (size_t)c.offsets[l] * c.F;
+        if (nc == 1) return (float)tl[idx[0] * (uint32_t)c.F + ff];
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a = fm::fma_(w[k], (float)tl[idx[k] * (uint32_t)c.F + ff], a);
+        return a;
+    }
+    if (type == ENC_ONEBLOB) {
+        const int nb = c.enc_param[e];
+        const float fnb = (float)nb;
+        const int d = r / nb, k = r % nb;
+        const float xv = pick(q_in, b + d);
+        const float left = oneblob_cdf((float)k / fnb, xv, fnb);
+        const float right = oneblob_cdf((float)(k + 1) / fnb, xv, fnb);
+        return right - left;
+    }
+    if (type == ENC_SH) {
+        const float sx = pick(q_in, b) * 2.0f - 1.0f, sy = pick(q_in, b + 1) * 2.0f - 1.0f,
+                    sz = pick(q_in, b + 2) * 2.0f - 1.0f;
+        if (r == 0) return 0.28209479177387814f;
+        if (r == 1) return -0.48860251190291987f * sy;
+        if (r == 2) return 0.48860251190291987f * sz;
+        return -0.48860251190291987f * sx;
+    }
+    return pick(q_in, b + r);
+}
+
+// The 8 features of chunk ch of one query into v[0..7].
+template <class T>
+__device__ __forceinline__ void encode_chunk(const DevCfg& c, const T* __restrict__ table, const float* q_in, int ch,
+                                             float v[8]) {
+    const int f0 = 8 * ch;
+    int e = 0;
+    for (int i = 1; i < c.n_enc; ++i)
+        if (f0 >= c.enc_offset[i]) e = i;
+    const int r0 = f0 - c.enc_offset[e];
+    const int b = c.enc_begin[e];
+    const int type = c.enc_type[e];
+    const bool inside = f0 + 8 <= c.enc_width && r0 + 8 <= (e + 1 < c.n_enc ? c.enc_offset[e + 1] : c.enc_width) - c.enc_offset[e];
+    if (f0 >= c.enc_width) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = 1.0f;
+        return;
+    }
+    if (inside && type == ENC_GRID && c.F == 2) {
+        const float x0 = pick(q_in, b), x1 = pick(q_in, b + 1), x2 = pick(q_in, b + 2);
+        const int l0 = r0 / 2;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) grid_level_features<T, 2>(c, table, l0 + i, x0, x1, x2, v + 2 * i);
+        return;
+    }
+    if (inside && type == ENC_GRID && c.F == 4) {
+        const float x0 = pick(q_in, b), x1 = pick(q_in, b + 1), x2 = pick(q_in, b + 2);
+        const int l0 = r0 / 4;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) grid_level_features<T, 4>(c, table, l0 + i, x0, x1, x2, v + 4 * i);
+        return;
+    }
+    if (inside && type == ENC_ONEBLOB && c.enc_param[e] >= 8) {
+        const int nb = c.enc_param[e];
+        const float fnb = (float)nb;
+        const int d = r0 / nb, k0 = r0 % nb;
+        const float xv = pick(q_in, b + d);
+        float left = oneblob_cdf((float)k0 / fnb, xv, fnb);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float right = oneblob_cdf((float)(k0 + i + 1) / fnb, xv, fnb);
+            v[i] = right - left;
+            left = right;
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = single_feature<T>(c, table, q_in, f0 + i);
 }
 
 }  // namespace dev

@@ -1,0 +1,251 @@
+// k_query_mfma.hip -- fused query kernel with the MLP on MFMA (BF16 or FP16
+// operands, FP32 accumulation), plus the packing kernels that produce its
+// weight fragments and half-precision grid tables.
+//
+// One wave handles 64 queries. Lane l = 32 h + j encodes, for queries j and
+// 32 + j of the tile, the feature chunks 2 kb + h (8 features each) of every
+// K-block kb of the first layer, and hands them to v_mfma_f32_32x32x16 as B
+// fragments; the encoding never leaves registers. Weights live in LDS in
+// fragment order (1 KB per 32x16 fragment, conflict-free reads). Hidden
+// layers chain register to register (see mfma.h). The output tile is split
+// between lanes l and l ^ 32 and exchanged with one shuffle, after which
+// lane l owns every raw output of query l and runs the FP32 mixture code
+// (normaliser, softmax, log-sum-exp and sampling stay FP32).
+#include "device.h"
+#include "kernels.h"
+#include "mfma.h"
+
+namespace npmg {
+namespace {
+
+using namespace dev;
+
+template <class T, int W, int K>
+__global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __restrict__ wfrag,
+                                                    const T* __restrict__ table, const QueryIO io, uint32_t n,
+                                                    uint32_t n_frag) {
+    using V = typename mf::Frag<T>::vec;
+    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    constexpr int TO = NOUT / 32;  // output tiles
+    constexpr int TH = W / 32;     // hidden tiles
+    constexpr int KBH = W / 16;    // K-blocks of a hidden layer's input
+    extern __shared__ uint4 smem_frag[];
+    for (uint32_t i = threadIdx.x; i < n_frag * 64u; i += 256u) smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
+    __syncthreads();
+    const T* sw = reinterpret_cast<const T*>(smem_frag);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 5, j = lane & 31;
+    const int KB0 = c.enc_padded / 16;
+    const uint32_t n_tiles = (n + 63u) / 64u;
+    for (uint32_t tile = blockIdx.x * 4u + (uint32_t)wave; tile < n_tiles; tile += gridDim.x * 4u) {
+        const uint32_t q0 = tile * 64u;
+        const uint32_t qa = min(q0 + (uint32_t)j, n - 1u), qb = min(q0 + 32u + (uint32_t)j, n - 1u);
+        float ina[16], inb[16];
+#pragma unroll
+        for (int d = 0; d < 16; ++d) {
+            ina[d] = d < c.n_in ? io.in[(size_t)d * n + qa] : 0.0f;
+            inb[d] = d < c.n_in ? io.in[(size_t)d * n + qb] : 0.0f;
+        }
+        mf::f32x16 acc[TH][2];
+#pragma unroll
+        for (int t = 0; t < TH; ++t) acc[t][0] = acc[t][1] = mf::f32x16{};
+        // Layer 0: encode chunk by chunk straight into B fragments.
+        for (int kb = 0; kb < KB0; ++kb) {
+            float va[8], vb[8];
+            encode_chunk<T>(c, table, ina, 2 * kb + h, va);
+            encode_chunk<T>(c, table, inb, 2 * kb + h, vb);
+            const V ba = mf::pack8<T>(va), bb = mf::pack8<T>(vb);
+#pragma unroll
+            for (int t = 0; t < TH; ++t) {
+                const V a = mf::load_frag<T>(sw, t * KB0 + kb, lane);
+                acc[t][0] = mf::Frag<T>::mma(a, ba, acc[t][0]);
+                acc[t][1] = mf::Frag<T>::mma(a, bb, acc[t][1]);
+            }
+        }
+        int fb = TH * KB0;
+        // Hidden layers: ReLU, round, and feed the accumulators back as B fragments.
+        for (int l = 1; l < c.n_layers - 1; ++l) {
+            V bx[2][KBH];
+#pragma unroll
+            for (int kb = 0; kb < KBH; ++kb)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    float v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float a = acc[kb >> 1][cc][8 * (kb & 1) + e];
+                        v[e] = a > 0.0f ? a : 0.0f;
+                    }
+                    bx[cc][kb] = mf::pack8<T>(v);
+                }
+#pragma unroll
+            for (int t = 0; t < TH; ++t) {
+                acc[t][0] = acc[t][1] = mf::f32x16{};
+#pragma unroll
+                for (int kb = 0; kb < KBH; ++kb) {
+                    const V a = mf::load_frag<T>(sw, fb + t * KBH + kb, lane);
+                    acc[t][0] = mf::Frag<T>::mma(a, bx[0][kb], acc[t][0]);
+                    acc[t][1] = mf::Frag<T>::mma(a, bx[1][kb], acc[t][1]);
+                }
+            }
+            fb += TH * KBH;
+        }
+        // Output layer.
+        V bx[2][KBH];
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                float v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float a = acc[kb >> 1][cc][8 * (kb & 1) + e];
+                    v[e] = a > 0.0f ? a : 0.0f;
+                }
+                bx[cc][kb] = mf::pack8<T>(v);
+            }
+        float out[NOUT];
+#pragma unroll
+        for (int t = 0; t < TO; ++t) {
+            mf::f32x16 o0 = {}, o1 = {};
+#pragma unroll
+            for (int kb = 0; kb < KBH; ++kb) {
+                const V a = mf::load_frag<T>(sw, fb + t * KBH + kb, lane);
+                o0 = mf::Frag<T>::mma(a, bx[0][kb], o0);
+                o1 = mf::Frag<T>::mma(a, bx[1][kb], o1);
+            }
+            // Lane (h, j) keeps the tile of its own query (c = h) and swaps the other with lane l ^ 32.
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float mine = h ? o1[r] : o0[r];
+                const float recv = __shfl_xor(h ? o0[r] : o1[r], 32);
+                const int a = r >> 2, b = r & 3;
+                out[32 * t + 8 * a + b] = h ? recv : mine;
+                out[32 * t + 8 * a + 4 + b] = h ? mine : recv;
+            }
+        }
+        const uint32_t q = q0 + (uint32_t)lane;
+        if (q < n) {
+            if (io.raw_out) {
+#pragma unroll
+                for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
+            }
+            if (io.u || io.n_eval > 0) {
+                Mixture<K> m;
+                build_mixture<K>(c, [&](int jj) { return out[jj]; }, m);
+                float th[K], ph[K];
+                if (io.normal) {
+                    const V3 nrm = mk(io.normal[q], io.normal[(size_t)n + q], io.normal[2 * (size_t)n + q]);
+                    apply_cosine<K>(c, nrm, m, false, th, ph);
+                }
+                if (io.u) {
+                    uint32_t lobe;
+                    const V3 d = mixture_sample<K>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
+                    io.out_dir[q] = d.x;
+                    io.out_dir[(size_t)n + q] = d.y;
+                    io.out_dir[2 * (size_t)n + q] = d.z;
+                    io.out_pdf[q] = mixture_pdf<K>(c, m, d);
+                    if (io.out_lobe) io.out_lobe[q] = lobe;
+                }
+                for (int e = 0; e < io.n_eval; ++e) {
+                    const float* ed = io.eval_dirs + (size_t)e * 3 * n;
+                    io.out_eval_pdf[(size_t)e * n + q] =
+                        mixture_pdf<K>(c, m, mk(ed[q], ed[(size_t)n + q], ed[2 * (size_t)n + q]));
+                }
+            }
+        }
+    }
+}
+
+// Weight fragments from full-precision row-major matrices: fragment
+// (layer, t, kb), lane (h, i), element e holds W_l[32 t + i][col] with
+// col = 16 kb + 8 h + e on the first layer and the chained permutation after.
+template <class T>
+__global__ void __launch_bounds__(256) k_pack_frags(const DevCfg c, const float* __restrict__ w, T* __restrict__ frag) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    uint32_t base = 0;
+    for (int l = 0; l < c.n_layers; ++l) {
+        const uint32_t iw = (uint32_t)c.layer_in[l], ow = (uint32_t)c.layer_out[l];
+        const uint32_t cnt = iw * ow;
+        if (gid < base + cnt) {
+            const uint32_t r = gid - base;
+            const uint32_t f = r / 512u, within = r % 512u;
+            const uint32_t lane = within / 8u, e = within % 8u;
+            const uint32_t kbs = iw / 16u;
+            const uint32_t t = f / kbs, kb = f % kbs;
+            const uint32_t h = lane / 32u, i = lane % 32u;
+            const uint32_t col = l == 0 ? 16u * kb + 8u * h + e : (uint32_t)mf::chained_column((int)kb, (int)h, (int)e);
+            frag[gid] = (T)w[c.layer_off[l] + (32u * t + i) * iw + col];
+            return;
+        }
+        base += cnt;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_to_half(const float* __restrict__ src, T* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) dst[i] = (T)src[i];
+}
+
+template <class T, int W, int K>
+hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const QueryIO& io, uint32_t n,
+                         int n_cu, hipStream_t s) {
+    const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
+    const uint32_t n_frag = n_mlp / 512u;
+    const size_t lds = (size_t)n_frag * 1024u;
+    auto kern = k_query_mfma<T, W, K>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const uint32_t tiles = (n + 63u) / 64u;
+    const uint32_t want = (tiles + 3u) / 4u;
+    const uint32_t grid = want < (uint32_t)(n_cu * per_cu) ? want : (uint32_t)(n_cu * per_cu);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, c, static_cast<const T*>(wfrag),
+                       static_cast<const T*>(table), io, n, n_frag);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, const QueryIO& io, uint32_t n,
+                            int n_cu, hipStream_t s) {
+    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, io, n, n_cu, s);
+    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, io, n, n_cu, s);
+    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, io, n, n_cu, s);
+    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, io, n, n_cu, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const QueryIO& io,
+                             uint32_t n, int n_cu, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (c.enc_padded % 16 != 0) return hipErrorInvalidValue;
+    if (precision == PREC_BF16) return launch_mfma_type<__bf16>(c, wfrag, table, io, n, n_cu, s);
+    if (precision == PREC_FP16) return launch_mfma_type<_Float16>(c, wfrag, table, io, n, n_cu, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s) {
+    const uint32_t n = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
+    const dim3 grid((n + 255) / 256);
+    if (precision == PREC_BF16) hipLaunchKernelGGL((k_pack_frags<__bf16>), grid, dim3(256), 0, s, c, w, (__bf16*)frag);
+    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_pack_frags<_Float16>), grid, dim3(256), 0, s, c, w, (_Float16*)frag);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (precision == PREC_BF16) hipLaunchKernelGGL((k_to_half<__bf16>), grid, dim3(256), 0, s, src, (__bf16*)dst, n);
+    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_to_half<_Float16>), grid, dim3(256), 0, s, src, (_Float16*)dst, n);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace npmg

@@ -1,0 +1,68 @@
+// mfma.h -- fragment layouts of the gfx950 32x32x16 MFMA and the network
+// pieces built on them (shared by the inference and training kernels).
+//
+// v_mfma_f32_32x32x16_{bf16,f16}: D[32][32] += A[32][16] * B[16][32], one
+// wave. Lane l = 32 h + j holds
+//   A: row j, k = 8h .. 8h+7        (8 values)
+//   B: column j, k = 8h .. 8h+7     (8 values)
+//   D: column j, rows 8a + 4h + b for a, b in 0..3 (16 floats, register 4a + b)
+//
+// Network layers use A = weights (rows = output neurons), B = activations
+// (columns = queries). A wave holds 64 queries as two 32-column tiles c = 0, 1.
+//
+// Layer chaining without data movement: the accumulator registers 8m .. 8m+7
+// (m = 0, 1) of output tile t are exactly the B fragment of K-block
+// kb = 2t + m of the next layer if the next layer's weight columns are
+// permuted so that logical input k = 16 kb + 8 h + e reads hidden neuron
+//   p(kb, h, e) = 32 (kb >> 1) + 8 (2 (kb & 1) + (e >> 2)) + 4 h + (e & 3).
+// The packing kernel applies this permutation to every layer after the first.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace npmg {
+namespace mf {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+
+// Hidden neuron read by logical input k = 16 kb + 8 h + e of a chained layer.
+__host__ __device__ __forceinline__ int chained_column(int kb, int h, int e) {
+    return 32 * (kb >> 1) + 8 * (2 * (kb & 1) + (e >> 2)) + 4 * h + (e & 3);
+}
+
+template <class T> struct Frag;
+template <> struct Frag<__bf16> {
+    using vec = bf16x8;
+    __device__ static __forceinline__ f32x16 mma(vec a, vec b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Frag<_Float16> {
+    using vec = f16x8;
+    __device__ static __forceinline__ f32x16 mma(vec a, vec b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// 8 floats -> one B/A fragment (round to nearest even).
+template <class T>
+__device__ __forceinline__ typename Frag<T>::vec pack8(const float* v) {
+    typename Frag<T>::vec r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (T)v[e];
+    return r;
+}
+
+// Fragment of the weights: 16 bytes per lane, fragments stored lane-major so
+// a wave reads 1 KB contiguously (conflict-free ds_read_b128).
+template <class T>
+__device__ __forceinline__ typename Frag<T>::vec load_frag(const T* base, int frag, int lane) {
+    const uint4 raw = reinterpret_cast<const uint4*>(base)[frag * 64 + lane];
+    return __builtin_bit_cast(typename Frag<T>::vec, raw);
+}
+
+}  // namespace mf
+}  // namespace npmg
