@@ -82,7 +82,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="all", choices=["all", "query"],
+                    help="query: time only the fused query kernel (for counter runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 1 GiB device copy 5 times (HBM-counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the guiding path would shard queries by index, see DESIGN.md)", file=sys.stderr)
@@ -128,6 +142,9 @@ def main():
     ms_q = per_prec[args.precision]
     qps = N_QUERIES / (ms_q * 1e-3)
 
+    if args.only == "query":
+        print(json.dumps({"ms_per_step": ms_q, "queries_per_s": qps, "precision": args.precision}))
+        return
     t = npmg.synth_train(77, N_TRAIN)
     loss = torch.empty(1, device="cuda")
 

@@ -26,8 +26,9 @@
 namespace npmg {
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
-hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const QueryIO& io,
-                             uint32_t n, int n_cu, hipStream_t s);
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
+                             const QueryIO& io, uint32_t n, int n_cu, hipStream_t s);
+bool mfma_split_grid(const DevCfg& c);
 hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
 }  // namespace npmg
@@ -148,6 +149,10 @@ struct npmg_ctx {
     TrainWS ws{};
     float* partials = nullptr;
     size_t partials_cap = 0;
+    void* sort_scratch = nullptr;  // deterministic grid-gradient mode
+    size_t sort_cap = 0;
+    void* genc = nullptr;  // level-major grid features [L F][n] of the MFMA query path
+    uint32_t genc_cap = 0;
     uint32_t step = 0;
     float loss_scale = 128.0f;
 
@@ -156,7 +161,8 @@ struct npmg_ctx {
     ~npmg_ctx() {
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
-                        (void*)partials, frag_train, frag_infer, htab_train, htab_infer})
+                        (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
+                        sort_scratch, genc})
             if (p) (void)hipFree(p);
     }
 
@@ -181,8 +187,19 @@ struct npmg_ctx {
     // Fused query with the training (0) or inference (1) parameters in the configured precision.
     void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s) {
         if (half()) {
+            void* g = nullptr;
+            if (mfma_split_grid(cfg.d)) {
+                if (n > genc_cap) {
+                    if (genc) hip_check(hipFree(genc), "hipFree");
+                    genc = nullptr;
+                    genc_cap = 0;
+                    genc = dalloc<uint16_t>((size_t)cfg.d.n_levels * cfg.d.F * n);
+                    genc_cap = n;
+                }
+                g = genc;
+            }
             hip_check(launch_query_mfma(cfg.d, cfg.precision, inference ? frag_infer : frag_train,
-                                        inference ? htab_infer : htab_train, io, n, n_cu, s),
+                                        inference ? htab_infer : htab_train, g, io, n, n_cu, s),
                       "query (mfma)");
         } else {
             const float* p = inference ? infer_params() : master;
@@ -193,7 +210,7 @@ struct npmg_ctx {
 
     void ensure_ws(uint32_t n) {
         if (n <= ws_cap) return;
-        for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss})
+        for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss, (void*)ws.dgrid})
             if (p) hip_check(hipFree(p), "hipFree");
         ws = TrainWS{};
         ws_cap = 0;
@@ -203,6 +220,15 @@ struct npmg_ctx {
         ws.g = dalloc<float>((size_t)d.n_hidden * d.width * n);
         ws.dout = dalloc<float>((size_t)d.n_out_pad * n);
         ws.loss = dalloc<float>(n);
+        if (d.grid_enc >= 0) ws.dgrid = dalloc<float>((size_t)d.n_levels * d.F * n);
+        if (cfg.deterministic_grid) {
+            const size_t need_sort = grid_scatter_sorted_bytes(d, n);
+            if (need_sort > sort_cap) {
+                if (sort_scratch) hip_check(hipFree(sort_scratch), "hipFree");
+                sort_scratch = dalloc<char>(need_sort);
+                sort_cap = need_sort;
+            }
+        }
         uint32_t splits, qps;
         const size_t need = wgrad_partials_needed(d, n, &splits, &qps);
         if (need > partials_cap) {
@@ -224,10 +250,16 @@ struct npmg_ctx {
 
     // Backward from ws.dout: parameter gradients into dparams (tcnn order).
     void train_backward(const float* in, float* dparams, float* denc, uint32_t n, hipStream_t s) {
-        if (cfg.n_grid) hip_check(launch_fill(dparams + cfg.n_mlp, 0.0f, cfg.n_grid, s), "fill");
-        hip_check(launch_train_backward_fp32(cfg.d, master, in, ws, denc, cfg.n_grid ? dparams + cfg.n_mlp : nullptr,
-                                             n, n_cu, s),
-                  "train backward");
+        hip_check(launch_train_backward_fp32(cfg.d, master, ws, denc, ws.dgrid, n, n_cu, s), "train backward");
+        if (cfg.n_grid) {
+            float* gg = dparams + cfg.n_mlp;
+            hip_check(launch_fill(gg, 0.0f, cfg.n_grid, s), "fill");
+            if (cfg.deterministic_grid)
+                hip_check(launch_grid_scatter_sorted(cfg.d, in, ws.dgrid, gg, n, sort_scratch, sort_cap, s),
+                          "grid scatter (sorted)");
+            else
+                hip_check(launch_grid_scatter_atomic(cfg.d, in, ws.dgrid, gg, n, s), "grid scatter");
+        }
         hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, s), "weight gradient");
     }
 

@@ -129,6 +129,12 @@ __device__ __forceinline__ uint32_t grid_index(const DevCfg& c, uint32_t hsize, 
         else if (c.hash_type == HASH_REVERSED_PRIME) { f0 = 2165219737u; f1 = 1434869437u; f2 = 2097192037u; }
         index = (p0 * f0) ^ (p1 * f1) ^ (p2 * f2);
     }
+    // index % hsize without a 32-bit division: hashed levels have power-of-two
+    // sizes; a dense index is below 2 hsize (cells reach resolution + 1 on the
+    // cube's faces), so one conditional subtraction is exact.
+    if ((hsize & (hsize - 1u)) == 0u) return index & (hsize - 1u);
+    if (index < hsize) return index;
+    if (index - hsize < hsize) return index - hsize;
     return index % hsize;
 }
 
@@ -172,9 +178,13 @@ __device__ __forceinline__ int level_corners(const DevCfg& c, int l, float x0, f
     return 8;
 }
 
-// tiny-cuda-nn common_device.h:533-548
+// tiny-cuda-nn common_device.h:533-548. The kernel's CDF is monotonic and
+// reaches its clamps; beyond |u| >= 1.0625 the clamped result is exactly 0 or
+// 1, so those evaluations are skipped (bit-identical).
 __device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
     const float u = x * inv_radius;
+    if (u <= -1.0625f) return 0.0f;
+    if (u >= 1.0625f) return 1.0f;
     const float u2 = u * u;
     const float u4 = u2 * u2;
     const float v = 15.0f / 16.0f * u * (1.0f - 2.0f / 3.0f * u2 + 1.0f / 5.0f * u4) + 0.5f;
@@ -275,15 +285,17 @@ __device__ __forceinline__ void encode_query(const DevCfg& c, const T* __restric
 // vMF mixture (src/render/guided/distribution.h, src/backend/src/util/vmf.h).
 template <int K>
 struct Mixture {
-    float lambda[K], kappa[K], lognorm[K], weight[K];
+    float lambda[K], kappa[K], norm[K], lognorm[K], weight[K];
     V3 mu[K];
     float total;
 };
 
-// log of the stable normaliser kappa / (2 pi (-expm1(-2 kappa))), 1/4pi at 0.
-__device__ __forceinline__ float stable_lognorm(float k) {
-    if (!(k > 1e-30f)) return fm::log(kInv4Pi);
-    return fm::log(k) - fm::log(kTwoPi) - fm::log(-fm::expm1(-2.0f * k));
+// The vMF normaliser with the e^{-kappa} of the exponent folded in,
+// C(k) = k e^{-k} / (4 pi sinh k) = k / (2 pi (-expm1(-2 k))), exactly 1/4pi at
+// k = 0; the pdf is C(k) exp(k (mu.w - 1)), evaluated as exp(k (mu.w - 1) + log C).
+__device__ __forceinline__ float stable_norm(float k) {
+    if (!(k > 1e-30f)) return kInv4Pi;
+    return k * kInv2Pi / -fm::expm1(-2.0f * k);
 }
 // Reference normaliser (vmf.h:22-27 and distribution.h:40-42) with its 1/4pi threshold.
 __device__ __forceinline__ float ref_norm(float k, float thresh) {
@@ -324,7 +336,8 @@ __device__ __forceinline__ void build_mixture(const DevCfg& c, Raw&& x, Mixture<
     for (int i = 0; i < K; ++i) {
         m.weight[i] = m.lambda[i] / m.total;
         m.mu[i] = sph_to_cart(theta[i], phi[i]);
-        m.lognorm[i] = stable_lognorm(m.kappa[i]);
+        m.norm[i] = stable_norm(m.kappa[i]);
+        m.lognorm[i] = fm::log(m.norm[i]);
     }
 }
 
@@ -339,7 +352,7 @@ __device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>
     cart_to_sph(nrm, &nt, &np);
     const float kb = c.cos_kappa;
     const V3 mu_b = sph_to_cart(nt, np);
-    const float norm_b = ref ? ref_norm(kb, 1e-3f) : fm::exp(stable_lognorm(kb));
+    const float norm_b = ref ? ref_norm(kb, 1e-3f) : stable_norm(kb);
     m.total = 0.0f;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -362,13 +375,15 @@ __device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>
             phi_out[i] = ph;
         }
         const float e = fm::exp(ka * (dot(nm, mu_a) - 1.0f) + kb * (dot(nm, mu_b) - 1.0f));
-        const float na = ref ? ref_norm(ka, 1e-3f) : fm::exp(stable_lognorm(ka));
-        const float nn = ref ? ref_norm(nk, 1e-3f) : fm::exp(stable_lognorm(nk));
+        const float na = ref ? ref_norm(ka, 1e-3f) : m.norm[i];
+        const float cn = stable_norm(nk);
+        const float nn = ref ? ref_norm(nk, 1e-3f) : cn;
         const float s = na * norm_b / nn * e;
         m.lambda[i] = s * m.lambda[i] * 1.0f;
         m.kappa[i] = nk;
         m.mu[i] = mu_new;
-        m.lognorm[i] = stable_lognorm(nk);
+        m.norm[i] = cn;
+        m.lognorm[i] = fm::log(cn);
         m.total += m.lambda[i];
     }
     m.total = smax(m.total, kEps);
@@ -617,11 +632,23 @@ __device__ __forceinline__ float single_feature(const DevCfg& c, const T* __rest
     return pick(q_in, b + r);
 }
 
-// The 8 features of chunk ch of one query into v[0..7].
+// The 8 features of chunk ch of one query into v[0..7]. When genc is given,
+// the grid features come from the level-major encoding [L F][n] (already in
+// the operand type) instead of being gathered here.
 template <class T>
 __device__ __forceinline__ void encode_chunk(const DevCfg& c, const T* __restrict__ table, const float* q_in, int ch,
-                                             float v[8]) {
+                                             float v[8], const T* __restrict__ genc = nullptr, uint32_t q = 0,
+                                             uint32_t n = 0) {
     const int f0 = 8 * ch;
+    if (genc) {
+        const int g0 = c.enc_offset[c.grid_enc], gw = c.n_levels * c.F;
+        if (f0 >= g0 && f0 + 8 <= g0 + gw) {
+            const T* src = genc + (size_t)(f0 - g0) * n + q;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = (float)src[(size_t)i * n];
+            return;
+        }
+    }
     int e = 0;
     for (int i = 1; i < c.n_enc; ++i)
         if (f0 >= c.enc_offset[i]) e = i;

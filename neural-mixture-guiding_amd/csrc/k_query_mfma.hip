@@ -20,10 +20,47 @@ namespace {
 
 using namespace dev;
 
+// Level-major grid encoding: grid (ceil(n / 256), n_levels). Blocks run in
+// linear order, so the whole GPU works on one level at a time and that
+// level's table slice (at most 2^19 x F entries) stays resident in each XCD's
+// L2 while it is gathered; the features go out in the operand type, [L F][n].
+template <class T>
+__global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __restrict__ table,
+                                                     const float* __restrict__ in, T* __restrict__ genc, uint32_t n) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    const int l = blockIdx.y;
+    if (q >= n) return;
+    const int b = c.enc_begin[c.grid_enc];
+    const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
+    if (c.F == 2) {
+        float f[2];
+        grid_level_features<T, 2>(c, table, l, x0, x1, x2, f);
+        genc[(size_t)(2 * l) * n + q] = (T)f[0];
+        genc[(size_t)(2 * l + 1) * n + q] = (T)f[1];
+    } else if (c.F == 4) {
+        float f[4];
+        grid_level_features<T, 4>(c, table, l, x0, x1, x2, f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) genc[(size_t)(4 * l + i) * n + q] = (T)f[i];
+    } else {
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+        const T* tl = table + (size_t)c.offsets[l] * c.F;
+        for (int ff = 0; ff < c.F; ++ff) {
+            float a = 0.0f;
+            if (nc == 1) a = (float)tl[idx[0] * (uint32_t)c.F + ff];
+            else
+                for (int k = 0; k < 8; ++k) a = fm::fma_(w[k], (float)tl[idx[k] * (uint32_t)c.F + ff], a);
+            genc[(size_t)(c.F * l + ff) * n + q] = (T)a;
+        }
+    }
+}
+
 template <class T, int W, int K>
 __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __restrict__ wfrag,
-                                                    const T* __restrict__ table, const QueryIO io, uint32_t n,
-                                                    uint32_t n_frag) {
+                                                    const T* __restrict__ table, const T* __restrict__ genc,
+                                                    const QueryIO io, uint32_t n, uint32_t n_frag) {
     using V = typename mf::Frag<T>::vec;
     constexpr int NOUT = (4 * K + 15) / 16 * 16;
     constexpr int TO = NOUT / 32;  // output tiles
@@ -52,8 +89,8 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __r
         // Layer 0: encode chunk by chunk straight into B fragments.
         for (int kb = 0; kb < KB0; ++kb) {
             float va[8], vb[8];
-            encode_chunk<T>(c, table, ina, 2 * kb + h, va);
-            encode_chunk<T>(c, table, inb, 2 * kb + h, vb);
+            encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
+            encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
             const V ba = mf::pack8<T>(va), bb = mf::pack8<T>(vb);
 #pragma unroll
             for (int t = 0; t < TH; ++t) {
@@ -189,8 +226,14 @@ __global__ void __launch_bounds__(256) k_to_half(const float* __restrict__ src, 
 }
 
 template <class T, int W, int K>
-hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const QueryIO& io, uint32_t n,
-                         int n_cu, hipStream_t s) {
+hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, void* genc, const QueryIO& io,
+                         uint32_t n, int n_cu, hipStream_t s) {
+    if (genc) {
+        hipLaunchKernelGGL((k_grid_encode<T>), dim3((n + 255) / 256, c.n_levels), dim3(256), 0, s, c,
+                           static_cast<const T*>(table), io.in, static_cast<T*>(genc), n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const uint32_t n_frag = n_mlp / 512u;
     const size_t lds = (size_t)n_frag * 1024u;
@@ -205,28 +248,33 @@ hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, 
     const uint32_t want = (tiles + 3u) / 4u;
     const uint32_t grid = want < (uint32_t)(n_cu * per_cu) ? want : (uint32_t)(n_cu * per_cu);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, c, static_cast<const T*>(wfrag),
-                       static_cast<const T*>(table), io, n, n_frag);
+                       static_cast<const T*>(table), static_cast<const T*>(genc), io, n, n_frag);
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, const QueryIO& io, uint32_t n,
-                            int n_cu, hipStream_t s) {
-    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, io, n, n_cu, s);
-    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, io, n, n_cu, s);
+hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, void* genc, const QueryIO& io,
+                            uint32_t n, int n_cu, hipStream_t s) {
+    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, genc, io, n, n_cu, s);
+    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, genc, io, n, n_cu, s);
+    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, genc, io, n, n_cu, s);
+    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, genc, io, n, n_cu, s);
     return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const QueryIO& io,
-                             uint32_t n, int n_cu, hipStream_t s) {
+bool mfma_split_grid(const DevCfg& c) {
+    return c.grid_enc >= 0 && c.enc_offset[c.grid_enc] % 8 == 0 && (c.n_levels * c.F) % 8 == 0;
+}
+
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
+                             const QueryIO& io, uint32_t n, int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (c.enc_padded % 16 != 0) return hipErrorInvalidValue;
-    if (precision == PREC_BF16) return launch_mfma_type<__bf16>(c, wfrag, table, io, n, n_cu, s);
-    if (precision == PREC_FP16) return launch_mfma_type<_Float16>(c, wfrag, table, io, n, n_cu, s);
+    if (genc && !mfma_split_grid(c)) return hipErrorInvalidValue;
+    if (precision == PREC_BF16) return launch_mfma_type<__bf16>(c, wfrag, table, genc, io, n, n_cu, s);
+    if (precision == PREC_FP16) return launch_mfma_type<_Float16>(c, wfrag, table, genc, io, n, n_cu, s);
     return hipErrorInvalidValue;
 }
 

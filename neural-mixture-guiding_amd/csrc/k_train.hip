@@ -113,15 +113,14 @@ __global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __
         [&](int j, float v) { dout[(size_t)j * n + q] = v; });
 }
 
-// Backward: g_{l-1} = (W_l^T g_l) masked by ReLU, from the output layer down;
-// then dL/denc = W_0^T g_0 and the grid scatter (tiny-cuda-nn
-// kernel_grid_backward, grid.h:386-509, atomic mode). w: row-major [out][in]
-// weights of every layer (the parameter vector), staged in LDS when they fit.
+// Backward: g_{l-1} = (W_l^T g_l) masked by ReLU, from the output layer down,
+// then dL/denc = W_0^T g_0; the grid rows of dL/denc go to dgrid [L F][n] for
+// the scatter kernels. w: row-major [out][in] weights of every layer (the
+// parameter vector), staged in LDS when they fit.
 template <int W, int NOUT, bool LDS_W>
 __global__ void __launch_bounds__(block_size<W>()) k_train_bwd(const DevCfg c, const float* __restrict__ w,
-                                                               const float* __restrict__ in, const TrainWS ws,
-                                                               float* __restrict__ denc,
-                                                               float* __restrict__ grid_grad, uint32_t n) {
+                                                               const TrainWS ws, float* __restrict__ denc,
+                                                               float* __restrict__ dgrid, uint32_t n) {
     constexpr int BS = block_size<W>();
     extern __shared__ float smem[];
     const uint32_t n_mlp = n_mlp_of(c);
@@ -156,41 +155,25 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_bwd(const DevCfg c, c
             }
             ow = W;
         }
-        // Layer 0: dL/denc for the grid features (and every feature when denc is requested).
+        // Layer 0: dL/denc for the grid features (kept for the level-major
+        // scatter) and for every feature when denc is requested.
         const int nin = c.layer_in[0];
         const int ge = c.grid_enc;
         const int gbeg = ge >= 0 ? c.enc_offset[ge] : 0;
         const int gend = ge >= 0 ? gbeg + c.n_levels * c.F : 0;
         const int jb = denc ? 0 : gbeg, je = denc ? nin : gend;
-        float px = 0.0f, py = 0.0f, pz = 0.0f;
-        if (ge >= 0) {
-            const int b = c.enc_begin[ge];
-            px = in[(size_t)b * n + q];
-            py = in[(size_t)(b + 1) * n + q];
-            pz = in[(size_t)(b + 2) * n + q];
-        }
-        int cached_l = -1, nc = 0;
-        uint32_t idx[8];
-        float wk[8];
         for (int j0 = jb & ~15; j0 < je; j0 += 16) {
             float acc[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
             // W_0 is [W][nin] row-major: row o, columns j0..j0+15 (nin is a multiple of 16).
             for (int o = 0; o < W; ++o) axpy_row<16>(sw + (size_t)o * nin + j0, gcol[o * BS], acc);
+#pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int jj = j0 + i;
                 if (jj < jb || jj >= je) continue;
                 if (denc) denc[(size_t)jj * n + q] = acc[i];
-                if (jj >= gbeg && jj < gend && grid_grad) {
-                    const int l = (jj - gbeg) / c.F, f = (jj - gbeg) % c.F;
-                    if (l != cached_l) {
-                        nc = level_corners(c, l, px, py, pz, idx, wk);
-                        cached_l = l;
-                    }
-                    float* gl = grid_grad + (size_t)c.offsets[l] * c.F + f;
-                    for (int k = 0; k < nc; ++k) atomicAdd(gl + (size_t)idx[k] * c.F, wk[k] * acc[i]);
-                }
+                if (dgrid && jj >= gbeg && jj < gend) dgrid[(size_t)(jj - gbeg) * n + q] = acc[i];
             }
         }
     }
@@ -331,8 +314,8 @@ hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, co
 }
 
 template <int W, int NOUT, bool LDS_W>
-hipError_t launch_bwd_t(const DevCfg& c, const float* w, const float* in, const TrainWS& ws, float* denc,
-                        float* grid_grad, uint32_t n, int n_cu, hipStream_t s) {
+hipError_t launch_bwd_t(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid, uint32_t n,
+                        int n_cu, hipStream_t s) {
     constexpr int BS = block_size<W>();
     constexpr int GR = W > NOUT ? W : NOUT;
     const uint32_t n_mlp = n_mlp_of(c);
@@ -342,7 +325,7 @@ hipError_t launch_bwd_t(const DevCfg& c, const float* w, const float* in, const 
     if (e != hipSuccess) return e;
     const uint32_t tiles = (n + BS - 1) / BS;
     const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
-    hipLaunchKernelGGL((k_train_bwd<W, NOUT, LDS_W>), dim3(grid), dim3(BS), lds, s, c, w, in, ws, denc, grid_grad, n);
+    hipLaunchKernelGGL((k_train_bwd<W, NOUT, LDS_W>), dim3(grid), dim3(BS), lds, s, c, w, ws, denc, dgrid, n);
     return hipGetLastError();
 }
 
@@ -364,16 +347,16 @@ hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const flo
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const float* in, const TrainWS& ws, float* denc,
-                                      float* grid_grad, uint32_t n, int n_cu, hipStream_t s) {
+hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
+                                      uint32_t n, int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const bool lds = fp32_weights_fit_lds(c);
-    if (c.width == 64 && c.n_out_pad == 32) return lds ? launch_bwd_t<64, 32, true>(c, w, in, ws, denc, grid_grad, n, n_cu, s)
-                                                       : launch_bwd_t<64, 32, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
-    if (c.width == 64 && c.n_out_pad == 64) return lds ? launch_bwd_t<64, 64, true>(c, w, in, ws, denc, grid_grad, n, n_cu, s)
-                                                       : launch_bwd_t<64, 64, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
-    if (c.width == 128 && c.n_out_pad == 32) return launch_bwd_t<128, 32, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
-    if (c.width == 128 && c.n_out_pad == 64) return launch_bwd_t<128, 64, false>(c, w, in, ws, denc, grid_grad, n, n_cu, s);
+    if (c.width == 64 && c.n_out_pad == 32) return lds ? launch_bwd_t<64, 32, true>(c, w, ws, denc, dgrid, n, n_cu, s)
+                                                       : launch_bwd_t<64, 32, false>(c, w, ws, denc, dgrid, n, n_cu, s);
+    if (c.width == 64 && c.n_out_pad == 64) return lds ? launch_bwd_t<64, 64, true>(c, w, ws, denc, dgrid, n, n_cu, s)
+                                                       : launch_bwd_t<64, 64, false>(c, w, ws, denc, dgrid, n, n_cu, s);
+    if (c.width == 128 && c.n_out_pad == 32) return launch_bwd_t<128, 32, false>(c, w, ws, denc, dgrid, n, n_cu, s);
+    if (c.width == 128 && c.n_out_pad == 64) return launch_bwd_t<128, 64, false>(c, w, ws, denc, dgrid, n, n_cu, s);
     return hipErrorInvalidValue;
 }
 

@@ -30,6 +30,7 @@ struct TrainWS {
     float* g;     // [n_hidden][width][n] dL/d(pre-activation) of each hidden layer
     float* dout;  // [n_out_pad][n]
     float* loss;  // [n]
+    float* dgrid; // [n_levels F][n] dL/d(grid features)
 };
 
 // FP32 fused query: encode -> MLP (FP32 VALU) -> mixture -> sample/pdf.
@@ -45,10 +46,19 @@ hipError_t launch_grid_corners(const DevCfg& c, const float* in, uint32_t* idx, 
 hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
                                      const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
                                      float ls_over_n, const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s);
-// Backward through the MLP and the grid: g of each hidden layer into ws.g,
-// dL/denc into denc (optional), grid gradient scattered into grid_grad.
-hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const float* in, const TrainWS& ws,
-                                      float* denc, float* grid_grad, uint32_t n, int n_cu, hipStream_t s);
+// Backward through the MLP: g of each hidden layer into ws.g, dL/d(grid
+// features) into dgrid [L F][n], dL/denc into denc (optional).
+hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
+                                      uint32_t n, int n_cu, hipStream_t s);
+// Grid-gradient scatter (tiny-cuda-nn kernel_grid_backward, grid.h:386-509):
+// atomic mode, one level per grid row so each level's slice stays in L2.
+hipError_t launch_grid_scatter_atomic(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
+                                      uint32_t n, hipStream_t s);
+// Deterministic mode: (table entry, contribution) pairs, stable radix sort by
+// entry, in-order sum of each run. ws_bytes reports/uses the scratch size.
+size_t grid_scatter_sorted_bytes(const DevCfg& c, uint32_t n);
+hipError_t launch_grid_scatter_sorted(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
+                                      uint32_t n, void* scratch, size_t scratch_bytes, hipStream_t s);
 // Weight gradients dW_l = sum_q g_l x_l^T, deterministic split-K reduction.
 hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,
                              uint32_t n, hipStream_t s);

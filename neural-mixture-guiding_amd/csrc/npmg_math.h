@@ -17,7 +17,7 @@
 //   log    x = m 2^e with m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(s),
 //          s = (m-1)/(m+1), odd series of atanh to s^11.
 //   log1p  Goldberg's correction log1p(x) = log(u) * x / (u - 1), u = 1 + x.
-//   expm1  Kahan's correction expm1(x) = (u - 1) * x / log(u), u = e^x.
+//   expm1  Taylor polynomial to degree 11 on |x| <= 3/4, e^x - 1 beyond.
 //   sin/cos  Cody & Waite reduction by pi/2 (three-part constant), Taylor
 //          polynomials on [-pi/4, pi/4] (sin to r^11, cos to r^12).
 //   atan   reduction |x|>1 -> pi/2 - atan(1/x) and, above tan(pi/12),
@@ -130,11 +130,21 @@ NPMG_HD float log1p(float x) {
 NPMG_HD float expm1(float x) {
     if (isnan_(x)) return x;
     if (x > 88.7228394f) return inf_();
-    const float u = exp(x);
-    if (u == 1.0f) return x;
-    const float um1 = u - 1.0f;
-    if (um1 == -1.0f) return -1.0f;
-    return um1 * (x / log(u));
+    if (abs_(x) <= 0.75f) {
+        // x (1 + x/2 + x^2/6 + ... + x^10/11!), Taylor to degree 11
+        float p = 1.0f / 39916800.0f;
+        p = fma_(p, x, 1.0f / 3628800.0f);
+        p = fma_(p, x, 1.0f / 362880.0f);
+        p = fma_(p, x, 1.0f / 40320.0f);
+        p = fma_(p, x, 1.0f / 5040.0f);
+        p = fma_(p, x, 1.0f / 720.0f);
+        p = fma_(p, x, 1.0f / 120.0f);
+        p = fma_(p, x, 1.0f / 24.0f);
+        p = fma_(p, x, 1.0f / 6.0f);
+        p = fma_(p, x, 0.5f);
+        return fma_(p * x, x, x);
+    }
+    return exp(x) - 1.0f;
 }
 
 NPMG_HD void sincos(float x, float* s_out, float* c_out) {

@@ -361,7 +361,8 @@ void mlp_one(const NetLayout& N, const float* w, const T* x0, T* raw, T* acts /*
 template <class T> struct Lobe {
     T lambda, kappa, theta, phi;
     V3<T> mu;      // cartesian mean direction
-    T lognorm;     // stable normaliser log(kappa / (2 pi (1 - e^{-2 kappa})))
+    T norm;        // stable normaliser C = kappa / (2 pi (-expm1(-2 kappa))), 1/4pi at 0
+    T lognorm;     // log C
 };
 
 // Reference-form normaliser kappa / (2 pi (1 - e^{-2 kappa})) (vmf.h:22-27)
@@ -369,10 +370,11 @@ template <class T> struct Lobe {
 template <class T> T ref_norm(T k, T thresh) {
     return k < thresh ? C<T>::inv_4pi : k * C<T>::inv_2pi / ((T)1 - M<T>::exp((T)-2 * k));
 }
-// Stable form: kappa / (2 pi (-expm1(-2 kappa))); exact limit 1/4pi at kappa = 0.
-template <class T> T stable_lognorm(T k) {
-    if (!(k > (T)1e-30)) return M<T>::log(C<T>::inv_4pi);
-    return M<T>::log(k) - M<T>::log(C<T>::two_pi) - M<T>::log(-M<T>::expm1((T)-2 * k));
+// Stable form C(k) = k e^{-k} / (4 pi sinh k) = k / (2 pi (-expm1(-2 k))), exact
+// limit 1/4pi at k = 0; the lobe pdf is C exp(k (mu.w - 1)).
+template <class T> T stable_norm(T k) {
+    if (!(k > (T)1e-30)) return C<T>::inv_4pi;
+    return k * C<T>::inv_2pi / -M<T>::expm1((T)-2 * k);
 }
 
 template <class T> struct Mixture {
@@ -412,7 +414,8 @@ template <class T> void build_mixture(const orc_cfg* c, const T* x /* stride 1 *
         Lobe<T>& L = m.lobe[i];
         m.weight[i] = L.lambda / m.total;
         L.mu = sph_to_cart(L.theta, L.phi);
-        L.lognorm = stable_lognorm(L.kappa);
+        L.norm = stable_norm(L.kappa);
+        L.lognorm = M<T>::log(L.norm);
     }
 }
 
@@ -423,7 +426,7 @@ template <class T> void apply_cosine(const orc_cfg* c, const float* nrm, Mixture
     cart_to_sph(v3<T>((T)nrm[0], (T)nrm[1], (T)nrm[2]), &nt, &np);
     const T kb = (T)c->cosine_kappa;
     const V3<T> mu_b = sph_to_cart(nt, np);
-    auto norm = [&](T k) { return ref ? ref_norm(k, (T)1e-3) : M<T>::exp(stable_lognorm(k)); };
+    auto norm = [&](T k) { return ref ? ref_norm(k, (T)1e-3) : stable_norm(k); };
     m.total = 0;
     for (int i = 0; i < m.K; ++i) {
         Lobe<T>& L = m.lobe[i];
@@ -439,7 +442,7 @@ template <class T> void apply_cosine(const orc_cfg* c, const float* nrm, Mixture
         T th, ph;
         cart_to_sph(nm, &th, &ph);
         const T e = M<T>::exp(L.kappa * (dot(nm, mu_a) - (T)1) + kb * (dot(nm, mu_b) - (T)1));
-        const T s = norm(L.kappa) * norm(kb) / norm(nk) * e;
+        const T s = (ref ? ref_norm(L.kappa, (T)1e-3) : L.norm) * norm(kb) / norm(nk) * e;
         L.lambda = s * L.lambda * (T)1;
         L.kappa = nk;
         L.theta = th;
@@ -447,7 +450,8 @@ template <class T> void apply_cosine(const orc_cfg* c, const float* nrm, Mixture
         // Reference numerics take the mean back through spherical coordinates;
         // the stable path keeps the normalised cartesian mean.
         L.mu = ref ? sph_to_cart(th, ph) : nm;
-        L.lognorm = stable_lognorm(nk);
+        L.norm = stable_norm(nk);
+        L.lognorm = M<T>::log(L.norm);
         m.total += L.lambda;
     }
     m.total = std::max(m.total, C<T>::eps);
