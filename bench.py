@@ -130,10 +130,27 @@ def main():
         log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
         return net, ms
 
-    per_prec = {}
+    t = npmg.synth_train(77, N_TRAIN)
+    loss = torch.empty(1, device="cuda")
+
+    def measure_train(netp, precision):
+        def train_step():
+            st = lib.npmg_train_step(netp._h, P(t["inputs"]), P(t["tgt_dir"]), P(t["tgt_rgb"]), P(t["tgt_pdf"]),
+                                     N_TRAIN, P(loss), stream.cuda_stream)
+            if st != 0:
+                raise npmg.NpmgError(st, lib.npmg_last_error(netp._h).decode())
+
+        ms = time_region(train_step, max(2, args.steps // 2), 2, stream)
+        log(f"{precision} train step: {ms:.3f} ms per 2^18 samples ({N_TRAIN / ms / 1e3:.2f} M samples/s), "
+            f"loss {loss.item():.4f}")
+        return ms
+
+    per_prec, per_prec_train = {}, {}
     for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
         netp, ms = measure_queries(prec)
         per_prec[prec] = ms
+        if args.only != "query":
+            per_prec_train[prec] = measure_train(netp, prec)
         if prec != args.precision:
             netp.close()
         else:
@@ -145,18 +162,8 @@ def main():
     if args.only == "query":
         print(json.dumps({"ms_per_step": ms_q, "queries_per_s": qps, "precision": args.precision}))
         return
-    t = npmg.synth_train(77, N_TRAIN)
-    loss = torch.empty(1, device="cuda")
-
-    def train_step():
-        st = lib.npmg_train_step(net._h, P(t["inputs"]), P(t["tgt_dir"]), P(t["tgt_rgb"]), P(t["tgt_pdf"]),
-                                 N_TRAIN, P(loss), stream.cuda_stream)
-        if st != 0:
-            raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
-
-    ms_t = time_region(train_step, max(2, args.steps // 2), 2, stream)
+    ms_t = per_prec_train[args.precision]
     sps = N_TRAIN / (ms_t * 1e-3)
-    log(f"train step: {ms_t:.3f} ms per 2^18 samples ({sps / 1e6:.2f} M samples/s), loss {loss.item():.4f}")
 
     table_bytes = net.n_grid * (4 if args.precision == "fp32" else 2)
     half = args.precision != "fp32"
@@ -189,9 +196,11 @@ def main():
         "mlp_compute": {"flops_per_launch": flops, "achieved_tflops": flops / (ms_q * 1e-3) / 1e12,
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
         "queries_per_s_by_precision": {k: N_QUERIES / (v * 1e-3) for k, v in per_prec.items()},
+        "train_samples_per_s_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_train.items()},
     }
     if "fp32" in per_prec and half:
         result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]
+        result[f"{args.precision}_train_speedup_vs_fp32"] = per_prec_train["fp32"] / per_prec_train[args.precision]
     if not args.no_cpu:
         params = net.get_params(1).cpu().numpy()
         q_host = {k: v.cpu().numpy() for k, v in q.items()}

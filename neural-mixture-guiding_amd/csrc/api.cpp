@@ -31,6 +31,18 @@ hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, 
 bool mfma_split_grid(const DevCfg& c);
 hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
+bool mfma_train_supported(const DevCfg& c);
+hipError_t launch_train_forward_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
+                                     const float* in, const TrainMfmaWS& ws, uint32_t n, int n_cu, hipStream_t s);
+hipError_t launch_train_backward_mfma(const DevCfg& c, int precision, const void* bfrag, const float* dout,
+                                      const TrainMfmaWS& ws, float* denc, uint32_t n, uint32_t* overflow, int n_cu,
+                                      hipStream_t s);
+hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& ws, float* partials,
+                             size_t partials_cap, float* dw, uint32_t n, uint32_t* overflow, hipStream_t s);
+size_t wgrad_mfma_partials_needed(const DevCfg& c, uint32_t n);
+size_t train_mask_words(const DevCfg& c, uint32_t n);
+size_t bfrag_elements(const DevCfg& c);
+hipError_t launch_pack_bfrags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 }  // namespace npmg
 
 using namespace npmg;
@@ -38,6 +50,8 @@ using namespace npmg;
 namespace {
 
 thread_local std::string g_create_error;
+
+constexpr float kMaxLossScale = 16777216.0f;  // upper bound of the dynamic loss scale, 2^24
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -151,10 +165,15 @@ struct npmg_ctx {
     size_t partials_cap = 0;
     void* sort_scratch = nullptr;  // deterministic grid-gradient mode
     size_t sort_cap = 0;
-    void* genc = nullptr;  // level-major grid features [L F][n] of the MFMA query path
+    void* genc = nullptr;  // level-major grid features [L F][n] of the MFMA paths
     uint32_t genc_cap = 0;
-    uint32_t step = 0;
-    float loss_scale = 128.0f;
+    // MFMA training: transposed weight fragments and the operand-type buffers
+    void* bfrag_train = nullptr;
+    TrainMfmaWS mws{};
+    // loss scale, overflow flag and step count on the device; decay^t table of the EMA debiasing
+    ScaleState* sstate = nullptr;
+    float* ema_pow = nullptr;
+    uint32_t ema_pow_len = 0;
 
     const float* infer_params() const { return ema ? ema : master; }
 
@@ -162,7 +181,8 @@ struct npmg_ctx {
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
-                        sort_scratch, genc})
+                        sort_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
+                        (void*)mws.raw, (void*)sstate, (void*)ema_pow})
             if (p) (void)hipFree(p);
     }
 
@@ -173,11 +193,27 @@ struct npmg_ctx {
         if (cfg.d.K != 8 && cfg.d.K != 16) throw std::invalid_argument("the fused kernels support 8 or 16 lobes");
     }
 
+    void check_train() const {
+        check_fused();
+        if (half() && !mfma_train_supported(cfg.d))
+            throw std::invalid_argument("MFMA training supports width 64 with 2-4 hidden layers or width 128 with 2-3");
+    }
+
+    void ensure_genc(uint32_t n) {
+        if (!mfma_split_grid(cfg.d) || n <= genc_cap) return;
+        if (genc) hip_check(hipFree(genc), "hipFree");
+        genc = nullptr;
+        genc_cap = 0;
+        genc = dalloc<uint16_t>((size_t)cfg.d.n_levels * cfg.d.F * n);
+        genc_cap = n;
+    }
+
     void refresh_layouts(hipStream_t s) {
         hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
         hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
         if (half()) {
             hip_check(launch_pack_frags(cfg.d, cfg.precision, master, frag_train, s), "pack");
+            if (bfrag_train) hip_check(launch_pack_bfrags(cfg.d, cfg.precision, master, bfrag_train, s), "pack");
             hip_check(launch_pack_frags(cfg.d, cfg.precision, infer_params(), frag_infer, s), "pack");
             hip_check(launch_to_half(cfg.precision, master + cfg.n_mlp, htab_train, cfg.n_grid, s), "table");
             hip_check(launch_to_half(cfg.precision, infer_params() + cfg.n_mlp, htab_infer, cfg.n_grid, s), "table");
@@ -187,19 +223,10 @@ struct npmg_ctx {
     // Fused query with the training (0) or inference (1) parameters in the configured precision.
     void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s) {
         if (half()) {
-            void* g = nullptr;
-            if (mfma_split_grid(cfg.d)) {
-                if (n > genc_cap) {
-                    if (genc) hip_check(hipFree(genc), "hipFree");
-                    genc = nullptr;
-                    genc_cap = 0;
-                    genc = dalloc<uint16_t>((size_t)cfg.d.n_levels * cfg.d.F * n);
-                    genc_cap = n;
-                }
-                g = genc;
-            }
+            ensure_genc(n);
             hip_check(launch_query_mfma(cfg.d, cfg.precision, inference ? frag_infer : frag_train,
-                                        inference ? htab_infer : htab_train, g, io, n, n_cu, s),
+                                        inference ? htab_infer : htab_train, mfma_split_grid(cfg.d) ? genc : nullptr,
+                                        io, n, n_cu, s),
                       "query (mfma)");
         } else {
             const float* p = inference ? infer_params() : master;
@@ -208,49 +235,96 @@ struct npmg_ctx {
         }
     }
 
-    void ensure_ws(uint32_t n) {
-        if (n <= ws_cap) return;
-        for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss, (void*)ws.dgrid})
+    void free_ws() {
+        for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss, (void*)ws.dgrid,
+                        mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask, (void*)mws.raw})
             if (p) hip_check(hipFree(p), "hipFree");
         ws = TrainWS{};
+        mws = TrainMfmaWS{};
         ws_cap = 0;
+    }
+
+    // Training workspace for n samples: FP32 activations for the FP32 path, the
+    // operand-type buffers and sign masks for the MFMA path.
+    void ensure_ws(uint32_t n) {
+        if (n <= ws_cap) return;
+        free_ws();
         const DevCfg& d = cfg.d;
-        ws.enc = dalloc<float>((size_t)d.enc_padded * n);
-        ws.acts = dalloc<float>((size_t)d.n_hidden * d.width * n);
-        ws.g = dalloc<float>((size_t)d.n_hidden * d.width * n);
+        const size_t hw = (size_t)d.n_hidden * d.width * n;
         ws.dout = dalloc<float>((size_t)d.n_out_pad * n);
         ws.loss = dalloc<float>(n);
         if (d.grid_enc >= 0) ws.dgrid = dalloc<float>((size_t)d.n_levels * d.F * n);
+        size_t need;
+        if (half()) {
+            mws.x0 = dalloc<uint16_t>((size_t)d.enc_padded * n);
+            mws.xh = dalloc<uint16_t>(hw);
+            mws.dyh = dalloc<uint16_t>(hw);
+            mws.dyo = dalloc<uint16_t>((size_t)d.n_out_pad * n);
+            mws.mask = dalloc<uint32_t>(train_mask_words(d, n));
+            mws.raw = dalloc<float>((size_t)d.n_out_pad * n);
+            mws.dgrid = ws.dgrid;
+            ensure_genc(n);
+            need = wgrad_mfma_partials_needed(d, n);
+        } else {
+            ws.enc = dalloc<float>((size_t)d.enc_padded * n);
+            ws.acts = dalloc<float>(hw);
+            ws.g = dalloc<float>(hw);
+            uint32_t splits, qps;
+            need = wgrad_partials_needed(d, n, &splits, &qps);
+        }
         if (cfg.deterministic_grid) {
             const size_t need_sort = grid_scatter_sorted_bytes(d, n);
             if (need_sort > sort_cap) {
                 if (sort_scratch) hip_check(hipFree(sort_scratch), "hipFree");
+                sort_scratch = nullptr;
+                sort_cap = 0;
                 sort_scratch = dalloc<char>(need_sort);
                 sort_cap = need_sort;
             }
         }
-        uint32_t splits, qps;
-        const size_t need = wgrad_partials_needed(d, n, &splits, &qps);
         if (need > partials_cap) {
             if (partials) hip_check(hipFree(partials), "hipFree");
+            partials = nullptr;
+            partials_cap = 0;
             partials = dalloc<float>(need);
             partials_cap = need;
         }
         ws_cap = n;
     }
 
-    // Forward with the training weights keeping activations (+ adjoint when targets are given).
-    void train_forward(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, float ls,
+    LossScale state_scale(uint32_t n) const { return LossScale{&sstate->scale, 1.0f / (float)n}; }
+
+    // Forward with the training weights keeping what the backward pass needs;
+    // with targets, the loss and dL/draw (scaled by ls) go to ws.loss / ws.dout.
+    void train_forward(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, LossScale ls,
                        hipStream_t s) {
         ensure_ws(n);
-        hip_check(launch_train_forward_fp32(cfg.d, wt_train, master + cfg.n_mlp, in, td, tr, tp, ls / (float)n, ws, n,
-                                            n_cu, s),
-                  "train forward");
+        if (half()) {
+            hip_check(launch_train_forward_mfma(cfg.d, cfg.precision, frag_train, htab_train, genc, in, mws, n, n_cu, s),
+                      "train forward (mfma)");
+            if (td) hip_check(launch_adjoint(cfg.d, mws.raw, td, tr, tp, ls, ws.dout, ws.loss, n, s), "adjoint");
+        } else {
+            hip_check(launch_train_forward_fp32(cfg.d, wt_train, master + cfg.n_mlp, in, td, tr, tp, ls, ws, n, n_cu, s),
+                      "train forward");
+        }
     }
 
-    // Backward from ws.dout: parameter gradients into dparams (tcnn order).
-    void train_backward(const float* in, float* dparams, float* denc, uint32_t n, hipStream_t s) {
-        hip_check(launch_train_backward_fp32(cfg.d, master, ws, denc, ws.dgrid, n, n_cu, s), "train backward");
+    // Backward from dL/draw (dout: ws.dout, or a caller buffer on the MFMA path):
+    // parameter gradients into dparams (tcnn order), dL/denc into denc when given.
+    // Non-finite gradients raise the overflow flag when `overflow` is set.
+    void train_backward(const float* in, const float* dout, float* dparams, float* denc, uint32_t n, bool overflow,
+                        hipStream_t s) {
+        uint32_t* of = overflow ? &sstate->overflow : nullptr;
+        if (half()) {
+            hip_check(launch_train_backward_mfma(cfg.d, cfg.precision, bfrag_train, dout, mws, denc, n, of, n_cu, s),
+                      "train backward (mfma)");
+        } else {
+            if (dout != ws.dout)
+                hip_check(hipMemcpyAsync(ws.dout, dout, (size_t)cfg.d.n_out_pad * n * sizeof(float),
+                                         hipMemcpyDeviceToDevice, s),
+                          "copy dout");
+            hip_check(launch_train_backward_fp32(cfg.d, master, ws, denc, ws.dgrid, n, n_cu, s), "train backward");
+        }
         if (cfg.n_grid) {
             float* gg = dparams + cfg.n_mlp;
             hip_check(launch_fill(gg, 0.0f, cfg.n_grid, s), "fill");
@@ -260,15 +334,25 @@ struct npmg_ctx {
             else
                 hip_check(launch_grid_scatter_atomic(cfg.d, in, ws.dgrid, gg, n, s), "grid scatter");
         }
-        hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, s), "weight gradient");
+        if (half())
+            hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, partials, partials_cap, dparams, n, of, s),
+                      "weight gradient (mfma)");
+        else
+            hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, of, s), "weight gradient");
     }
 
+    // Adam + EMA. ls > 0: the gradient is divided by ls (split-pass API);
+    // ls == 0: by the device loss scale, and a raised overflow flag skips the step.
     void optimizer_step(const float* dparams, float ls, hipStream_t s) {
-        ++step;
         AdamArgs a{};
         a.n = cfg.n_params;
         a.n_matrix = cfg.n_mlp;
         a.loss_scale = ls;
+        a.scale_from_state = !(ls > 0.0f);
+        a.check_overflow = a.scale_from_state && cfg.dynamic_scale;
+        a.st = sstate;
+        a.ema_pow = ema_pow;
+        a.ema_pow_len = ema_pow_len;
         a.lr = cfg.lr;
         a.beta1 = cfg.beta1;
         a.beta2 = cfg.beta2;
@@ -276,9 +360,8 @@ struct npmg_ctx {
         a.l2_reg = cfg.l2_reg;
         a.ema = ema != nullptr;
         a.ema_decay = cfg.ema_decay;
-        a.ema_debias_old = 1 - (float)std::pow(cfg.ema_decay, step - 1);
-        a.ema_debias_new = 1.0f / (1 - (float)std::pow(cfg.ema_decay, step));
         hip_check(launch_adam(a, master, dparams, m, v, steps, ema, s), "adam");
+        hip_check(launch_scale_update(sstate, a.check_overflow, cfg.scale_window, kMaxLossScale, s), "loss scale");
         refresh_layouts(s);
     }
 };
@@ -335,7 +418,6 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
         const Config& c = ctx->cfg;
-        ctx->loss_scale = c.loss_scale;
         ctx->master = dalloc<float>(c.n_params);
         if (c.ema) ctx->ema = dalloc<float>(c.n_params);
         ctx->m = dalloc<float>(c.n_params);
@@ -350,7 +432,22 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
             ctx->frag_infer = dalloc<uint16_t>(c.n_mlp);
             ctx->htab_train = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
             ctx->htab_infer = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
+            if (mfma_train_supported(c.d)) ctx->bfrag_train = dalloc<uint16_t>(bfrag_elements(c.d));
         }
+        ctx->sstate = dalloc<ScaleState>(1);
+        const ScaleState st0{c.loss_scale, 0u, 0u, 0u};
+        hip_check(hipMemcpy(ctx->sstate, &st0, sizeof(st0), hipMemcpyHostToDevice), "upload");
+        // decay^t for the EMA debiasing (tcnn ema.h:110-111 evaluates std::pow in
+        // double): tabulated until 1 - decay^t rounds to 1 in FP32.
+        std::vector<float> pw;
+        for (uint32_t t = 0; t < (1u << 22); ++t) {
+            const float v = (float)std::pow((double)c.ema_decay, (double)t);
+            pw.push_back(v);
+            if (1.0f - v == 1.0f) break;
+        }
+        ctx->ema_pow_len = (uint32_t)pw.size();
+        ctx->ema_pow = dalloc<float>(pw.size());
+        hip_check(hipMemcpy(ctx->ema_pow, pw.data(), pw.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
         hip_check(hipMemset(ctx->m, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->v, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->steps, 0, c.n_params * sizeof(uint32_t)), "memset");
@@ -525,8 +622,8 @@ npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* t
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
         need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && dout && loss), "null buffer");
-        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf, n ? loss_scale / (float)n : 0.0f, dout,
-                                 loss, n, S(stream)),
+        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf,
+                                 LossScale{nullptr, n ? loss_scale / (float)n : 0.0f}, dout, loss, n, S(stream)),
                   "adjoint");
     });
 }
@@ -535,13 +632,10 @@ npmg_status npmg_backward(npmg_ctx* ctx, const float* in, const float* dout, uin
                           npmg_stream stream) {
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
-        ctx->check_fused();
+        ctx->check_train();
         need(n > 0 && in && dout && dparams, "null buffer or empty batch");
-        ctx->train_forward(in, nullptr, nullptr, nullptr, n, 1.0f, S(stream));
-        hip_check(hipMemcpyAsync(ctx->ws.dout, dout, (size_t)ctx->cfg.d.n_out_pad * n * sizeof(float),
-                                 hipMemcpyDeviceToDevice, S(stream)),
-                  "copy dout");
-        ctx->train_backward(in, dparams, denc, n, S(stream));
+        ctx->train_forward(in, nullptr, nullptr, nullptr, n, LossScale{nullptr, 1.0f}, S(stream));
+        ctx->train_backward(in, dout, dparams, denc, n, false, S(stream));
     });
 }
 
@@ -558,22 +652,25 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
                             const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream) {
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
-        ctx->check_fused();
+        ctx->check_train();
         need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
         hipStream_t s = S(stream);
-        const float ls = ctx->loss_scale;
-        ctx->train_forward(in, tgt_dir, tgt_rgb, tgt_pdf, n, ls, s);
-        ctx->train_backward(in, ctx->grad, nullptr, n, s);
-        ctx->optimizer_step(ctx->grad, ls, s);
+        ctx->train_forward(in, tgt_dir, tgt_rgb, tgt_pdf, n, ctx->state_scale(n), s);
+        ctx->train_backward(in, ctx->ws.dout, ctx->grad, nullptr, n, ctx->cfg.dynamic_scale, s);
+        ctx->optimizer_step(ctx->grad, 0.0f, s);
         if (loss_out) hip_check(launch_mean(ctx->ws.loss, n, loss_out, s), "loss mean");
     });
 }
 
 npmg_status npmg_train_state(const npmg_ctx* ctx, float* loss_scale, uint32_t* step) {
     if (!ctx) return NPMG_EINVAL;
-    if (loss_scale) *loss_scale = ctx->loss_scale;
-    if (step) *step = ctx->step;
-    return NPMG_OK;
+    return guarded(const_cast<npmg_ctx*>(ctx), [&] {
+        ScaleState st{};
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        hip_check(hipMemcpy(&st, ctx->sstate, sizeof(st), hipMemcpyDeviceToHost), "copy state");
+        if (loss_scale) *loss_scale = st.scale;
+        if (step) *step = st.step;
+    });
 }
 
 npmg_status npmg_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,

@@ -264,6 +264,20 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
 
 }  // namespace
 
+hipError_t launch_grid_encode(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
+                              uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255) / 256, c.n_levels);
+    if (precision == PREC_BF16)
+        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, in, (__bf16*)genc, n);
+    else if (precision == PREC_FP16)
+        hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, in,
+                           (_Float16*)genc, n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 bool mfma_split_grid(const DevCfg& c) {
     return c.grid_enc >= 0 && c.enc_offset[c.grid_enc] % 8 == 0 && (c.n_levels * c.F) % 8 == 0;
 }

@@ -105,7 +105,7 @@ size_t grid_scatter_sorted_bytes(const DevCfg& c, uint32_t n) {
     if (c.grid_enc < 0) return 0;
     const size_t np = (size_t)n * c.n_levels * 8;
     size_t tmp = 0;
-    rocprim::radix_sort_pairs((void*)nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+    (void)rocprim::radix_sort_pairs((void*)nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                               (uint32_t*)nullptr, np, 0, key_bits(c) + 1, (hipStream_t)0);
     return np * 4 * 4 + np * c.F * 4 + ((tmp + 255) / 256) * 256 + 1024;
 }

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
                                                                const float* __restrict__ in,
                                                                const float* __restrict__ tgt_dir,
                                                                const float* __restrict__ tgt_rgb,
-                                                               const float* __restrict__ tgt_pdf, float ls_over_n,
+                                                               const float* __restrict__ tgt_pdf, const LossScale ls,
                                                                const TrainWS ws, uint32_t n) {
     constexpr int BS = block_size<W>();
     constexpr int NOUT = (4 * K + 15) / 16 * 16;
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
         if (tgt_dir) {
             const float loss = mixture_adjoint<K>(
                 c, [&](int jj) { return out[jj]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls_over_n,
+                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.value(),
                 [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
             ws.loss[q] = loss;
         }
@@ -103,13 +103,13 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
 template <int K>
 __global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __restrict__ raw,
                                                  const float* __restrict__ tgt_dir, const float* __restrict__ tgt_rgb,
-                                                 const float* __restrict__ tgt_pdf, float ls_over_n,
+                                                 const float* __restrict__ tgt_pdf, const LossScale ls,
                                                  float* __restrict__ dout, float* __restrict__ loss, uint32_t n) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     if (q >= n) return;
     loss[q] = mixture_adjoint<K>(
         c, [&](int j) { return raw[(size_t)j * n + q]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls_over_n,
+        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.value(),
         [&](int j, float v) { dout[(size_t)j * n + q] = v; });
 }
 
@@ -235,12 +235,15 @@ __global__ void __launch_bounds__(256) k_wgrad(const DevCfg c, const WgradTiles 
 }
 
 __global__ void __launch_bounds__(256) k_reduce_splits(const float* __restrict__ partials, uint32_t n_splits,
-                                                       uint32_t len, float* __restrict__ out) {
+                                                       uint32_t len, float* __restrict__ out,
+                                                       uint32_t* __restrict__ overflow) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= len) return;
     float s = 0.0f;
-    for (uint32_t k = 0; k < n_splits; ++k) s += partials[(size_t)k * len + i];
-    out[i] = s;
+    if (i < len) {
+        for (uint32_t k = 0; k < n_splits; ++k) s += partials[(size_t)k * len + i];
+        out[i] = s;
+    }
+    if (overflow && !__all(__builtin_isfinite(s) ? 1 : 0) && (threadIdx.x & 63u) == 0) atomicOr(overflow, 1u);
 }
 
 // tiny-cuda-nn optimizers/adam.h:47-119 followed by the EMA of
@@ -251,7 +254,8 @@ __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restric
                                               float* __restrict__ ema) {
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
     if (i >= a.n) return;
-    float g = grad[i] / a.loss_scale;
+    if (a.check_overflow && a.st->overflow) return;
+    float g = grad[i] / (a.scale_from_state ? a.st->scale : a.loss_scale);
     const bool skip = i >= a.n_matrix && g == 0.0f;
     float w = master[i];
     if (!skip) {
@@ -268,7 +272,27 @@ __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restric
         w = w - eff * m1;
         master[i] = w;
     }
-    if (a.ema) ema[i] = (ema[i] * a.ema_decay * a.ema_debias_old + w * (1 - a.ema_decay)) * a.ema_debias_new;
+    if (a.ema) {
+        const uint32_t t = a.st->step + 1u, last = a.ema_pow_len - 1u;
+        const float deb_old = 1 - a.ema_pow[t - 1u < last ? t - 1u : last];
+        const float deb_new = 1.0f / (1 - a.ema_pow[t < last ? t : last]);
+        ema[i] = (ema[i] * a.ema_decay * deb_old + w * (1 - a.ema_decay)) * deb_new;
+    }
+}
+
+__global__ void k_scale_update(ScaleState* st, bool dynamic, uint32_t window, float max_scale) {
+    if (threadIdx.x != 0) return;
+    if (dynamic && st->overflow) {
+        st->scale = st->scale * 0.5f < 1.0f ? 1.0f : st->scale * 0.5f;
+        st->good = 0;
+    } else {
+        st->step += 1u;
+        if (dynamic && ++st->good >= window) {
+            st->scale = st->scale * 2.0f > max_scale ? max_scale : st->scale * 2.0f;
+            st->good = 0;
+        }
+    }
+    st->overflow = 0;
 }
 
 __global__ void __launch_bounds__(256) k_transpose_mlp(const DevCfg c, const float* __restrict__ w,
@@ -298,7 +322,7 @@ __global__ void __launch_bounds__(1024) k_mean(const float* __restrict__ x, uint
 
 template <int W, int K, bool LDS_W>
 hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, const float* in, const float* td,
-                        const float* tr, const float* tp, float ls, const TrainWS& ws, uint32_t n, int n_cu,
+                        const float* tr, const float* tp, LossScale ls, const TrainWS& ws, uint32_t n, int n_cu,
                         hipStream_t s) {
     constexpr int BS = block_size<W>();
     const uint32_t n_mlp = n_mlp_of(c);
@@ -334,7 +358,7 @@ hipError_t launch_bwd_t(const DevCfg& c, const float* w, const TrainWS& ws, floa
 bool fp32_weights_fit_lds(const DevCfg& c);
 
 hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
-                                     const float* td, const float* tr, const float* tp, float ls, const TrainWS& ws,
+                                     const float* td, const float* tr, const float* tp, LossScale ls, const TrainWS& ws,
                                      uint32_t n, int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const bool lds = fp32_weights_fit_lds(c);
@@ -369,9 +393,11 @@ size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, ui
     return (size_t)splits * n_mlp_of(c);
 }
 
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
+
 hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,
-                             uint32_t n, hipStream_t s) {
-    if (n == 0) return hipMemsetAsync(dw, 0, n_mlp_of(c) * sizeof(float), s);
+                             uint32_t n, uint32_t* overflow, hipStream_t s) {
+    if (n == 0) return launch_fill(dw, 0.0f, n_mlp_of(c), s);
     WgradTiles t{};
     for (int l = 0; l < c.n_layers; ++l)
         for (int o0 = 0; o0 < c.layer_out[l]; o0 += 64)
@@ -386,7 +412,19 @@ hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t len = n_mlp_of(c);
-    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, splits, len, dw);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, splits, len, dw, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_splits(const float* partials, uint32_t n_splits, uint32_t len, float* out, hipStream_t s,
+                                uint32_t* overflow) {
+    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, n_splits, len, out,
+                       overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_update(ScaleState* st, bool dynamic, uint32_t window, float max_scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_update, dim3(1), dim3(64), 0, s, st, dynamic, window, max_scale);
     return hipGetLastError();
 }
 
@@ -409,7 +447,7 @@ hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s) {
 }
 
 hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* td, const float* tr, const float* tp,
-                          float ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
+                          LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256);
     if (c.K == 8) hipLaunchKernelGGL((k_adjoint<8>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);

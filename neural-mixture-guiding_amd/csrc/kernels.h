@@ -33,6 +33,34 @@ struct TrainWS {
     float* dgrid; // [n_levels F][n] dL/d(grid features)
 };
 
+// Loss scale divided by the batch size, as seen by the adjoint: a device scalar
+// (dynamic loss scaling) times a host factor, or the host factor alone.
+struct LossScale {
+    const float* dev;  // device scalar or null
+    float mul;
+    __device__ float value() const { return (dev ? dev[0] : 1.0f) * mul; }
+};
+
+// Device-resident optimiser/loss-scale state, so the train step needs no host
+// round trip (and can be captured in a graph).
+struct ScaleState {
+    float scale;        // current loss scale
+    uint32_t good;      // finite steps since the scale last changed
+    uint32_t overflow;  // raised by the backward kernels when a gradient is not finite
+    uint32_t step;      // optimiser steps applied (EMA debiasing)
+};
+
+// Training buffers of the MFMA path (T = the operand type, [dim][n] unless noted).
+struct TrainMfmaWS {
+    void* x0;        // [enc_padded][n] layer-0 inputs (T)
+    void* xh;        // [H][W][n] inputs of layers 1..H, post-ReLU (T)
+    void* dyh;       // [H][W][n] gradients w.r.t. the pre-activations of hidden layers 0..H-1 (T)
+    void* dyo;       // [n_out_pad][n] gradients w.r.t. the raw outputs (T)
+    uint32_t* mask;  // [ceil(n / 64)][H][W / 32][64] ReLU sign bits in accumulator order
+    float* raw;      // [n_out_pad][n] raw outputs (FP32)
+    float* dgrid;    // [L F][n] dL/d(grid features) (FP32)
+};
+
 // FP32 fused query: encode -> MLP (FP32 VALU) -> mixture -> sample/pdf.
 // wt: MLP weights transposed per layer ([in][out]); table: grid parameters.
 hipError_t launch_query_fp32(const DevCfg& c, const float* wt, const float* table, const QueryIO& io, uint32_t n,
@@ -45,7 +73,7 @@ hipError_t launch_grid_corners(const DevCfg& c, const float* in, uint32_t* idx, 
 // Training (FP32): forward storing activations (+ adjoint when tgt != null).
 hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
                                      const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
-                                     float ls_over_n, const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s);
+                                     LossScale ls, const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s);
 // Backward through the MLP: g of each hidden layer into ws.g, dL/d(grid
 // features) into dgrid [L F][n], dL/denc into denc (optional).
 hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
@@ -60,23 +88,34 @@ size_t grid_scatter_sorted_bytes(const DevCfg& c, uint32_t n);
 hipError_t launch_grid_scatter_sorted(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
                                       uint32_t n, void* scratch, size_t scratch_bytes, hipStream_t s);
 // Weight gradients dW_l = sum_q g_l x_l^T, deterministic split-K reduction.
+// Non-finite sums raise *overflow when it is given.
 hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,
-                             uint32_t n, hipStream_t s);
+                             uint32_t n, uint32_t* overflow, hipStream_t s);
 // Adam (+EMA) over all parameters.
+// The gradient is divided by the loss scale (st->scale, or loss_scale without
+// state); with check_overflow a raised st->overflow skips the whole step. The
+// EMA debiasing reads decay^t from ema_pow[min(t, ema_pow_len - 1)] with
+// t = st->step + 1 (tcnn computes std::pow in double on the host).
 struct AdamArgs {
     size_t n, n_matrix;
     float loss_scale, lr, beta1, beta2, eps, l2_reg;
-    bool ema;
-    float ema_decay, ema_debias_old, ema_debias_new;
+    bool ema, scale_from_state, check_overflow;
+    float ema_decay;
+    const ScaleState* st;
+    const float* ema_pow;
+    uint32_t ema_pow_len;
 };
 hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, float* m, float* v, uint32_t* steps,
                        float* ema, hipStream_t s);
+// After the optimiser: counts the step and, with dynamic scaling, halves the
+// scale on overflow or doubles it after `window` finite steps; clears the flag.
+hipError_t launch_scale_update(ScaleState* st, bool dynamic, uint32_t window, float max_scale, hipStream_t s);
 // Transposes the MLP weights of every layer ([out][in] -> [in][out]).
 hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipStream_t s);
 // Mean of n values into out[0] (deterministic tree order).
 hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s);
 hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
-                          const float* tgt_pdf, float ls_over_n, float* dout, float* loss, uint32_t n, hipStream_t s);
+                          const float* tgt_pdf, LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s);
 
 hipError_t launch_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,
                                 float* eval_dirs, hipStream_t s);

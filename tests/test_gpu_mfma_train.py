@@ -1,0 +1,236 @@
+"""MFMA (BF16/FP16 operands, FP32 accumulation) training path.
+
+The backward chain and weight gradients are checked three ways:
+  * against a numpy emulation of the same rounding points (grid table, encoded
+    features, weights, hidden activations and every layer's output gradient
+    rounded to the operand type, everything else in FP64), which leaves only
+    FP32 accumulation order: the parity bar, 2e-4 relative in the 2-norm;
+  * against FP64 with unrounded operands and the same ReLU masks (the oracle's
+    backward is pinned by central finite differences in the CPU suite): the
+    cost of the operand precision, bounded by 1e-2 relative;
+  * against FP64 with its own masks, which adds the units whose
+    pre-activation rounds across zero: reported, with a loose bound.
+Dynamic loss scaling is driven from the host: targets scaled by 1e6 overflow
+the FP16 operands, which must skip the step and halve the scale.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _round(a, prec):
+    import torch
+
+    if prec == "fp16":
+        return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+def _configs(prec):
+    import npmg
+
+    ref = npmg.reference_config()
+    ref["precision"] = prec
+    return {
+        "small": npmg.benchmark_config("small", precision=prec),
+        "full": npmg.benchmark_config("full", precision=prec),
+        "reference": ref,
+        "k16": npmg.benchmark_config("small", precision=prec, n_lobes=16),
+        "w128": npmg.benchmark_config("small", precision=prec, width=128),
+    }
+
+
+def _layers(oc, p, enc_w, out_w):
+    layers, off, n_in = [], 0, enc_w
+    for l in range(oc.n_hidden_layers + 1):
+        n_out = oc.width if l < oc.n_hidden_layers else out_w
+        layers.append(p[off:off + n_out * n_in].reshape(n_out, n_in).astype(np.float64))
+        off += n_out * n_in
+        n_in = oc.width
+    return layers
+
+
+def _chain(oracle, oc, net, p, inp, prec=None, masks=None):
+    """Forward keeping every layer's input and ReLU mask. prec: round at the
+    kernels' points (table, features, weights, activations), None: FP64 with
+    unrounded operands. masks: ReLU masks to impose instead of the signs."""
+    rnd = (lambda a: _round(a, prec)) if prec else (lambda a: np.asarray(a, np.float64))
+    table = p[net.n_mlp:]
+    if prec:
+        enc = oracle.encode(oc, rnd(table).astype(np.float32), inp)
+    else:
+        enc = oracle.encode(oc, table, inp, dtype=np.float64)
+    W = [rnd(w) for w in _layers(oc, p, net.encoded_width, net.padded_output)]
+    xs, ms = [rnd(enc)], []
+    for l, w in enumerate(W):
+        z = w @ xs[-1]
+        if l < len(W) - 1:
+            m = (z > 0) if masks is None else masks[l]
+            ms.append(m)
+            xs.append(rnd(np.where(m, z, 0.0)))
+    return {"W": W, "xs": xs, "masks": ms, "raw": z, "rnd": rnd}
+
+
+def _backward(ch, dout):
+    """Weight gradients (flat, tcnn order) and dL/denc of a chain from _chain."""
+    W, xs, ms, rnd = ch["W"], ch["xs"], ch["masks"], ch["rnd"]
+    dy = rnd(dout)
+    dws = [None] * len(W)
+    for l in range(len(W) - 1, -1, -1):
+        dws[l] = dy @ xs[l].T
+        g = W[l].T @ dy
+        if l > 0:
+            dy = rnd(np.where(ms[l - 1], g, 0.0))
+    return np.concatenate([d.ravel() for d in dws]), g
+
+
+def _rel2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# 2-norm relative error bounds against FP64 with unrounded operands and the
+# network's own ReLU signs: units whose pre-activation is within rounding of
+# zero switch sides, each flip contributing a whole term, so this error grows
+# like sqrt(eps) (measured: bf16 4-8e-2, fp16 1.5-3e-2 for random dL/draw).
+FLIP_BOUND = {"bf16": 0.15, "fp16": 0.06}
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("name", ["small", "full", "reference", "k16", "w128"])
+def test_mfma_backward(gpu, oracle, prec, name):
+    import torch
+    import npmg
+    import pyoracle
+
+    cfg = _configs(prec)[name]
+    net = npmg.GuidingNetwork(cfg, seed=7272)
+    oc = pyoracle.make_cfg(cfg)
+    n = 2000  # not a multiple of 64: partial last tile
+    q = npmg.synth_queries(8, n)
+    rng = np.random.default_rng(3)
+    dout = rng.normal(size=(net.padded_output, n)).astype(np.float32)
+    dparams, denc = net.backward(q["inputs"], torch.from_numpy(dout).cuda(), want_denc=True)
+    g = _np(dparams)
+    p = _np(net.get_params(0))
+    inp = _np(q["inputs"])
+    # 1) the kernels' own rounding points: only accumulation order remains
+    emu = _chain(oracle, oc, net, p, inp, prec)
+    dw_e, denc_e = _backward(emu, dout)
+    gg_e = oracle.grid_backward(oc, inp, denc_e)
+    e_w, e_enc, e_grid = _rel2(g[:net.n_mlp], dw_e), _rel2(_np(denc), denc_e), _rel2(g[net.n_mlp:], gg_e)
+    # 2) FP64, unrounded operands, same ReLU masks: the arithmetic precision alone
+    ref_m = _chain(oracle, oc, net, p, inp, None, masks=emu["masks"])
+    dw_m, denc_m = _backward(ref_m, dout)
+    m_w, m_grid = _rel2(g[:net.n_mlp], dw_m), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_m))
+    # 3) FP64 with its own masks (kink flips included)
+    ref = _chain(oracle, oc, net, p, inp, None)
+    dw_r, denc_r = _backward(ref, dout)
+    f_w, f_grid = _rel2(g[:net.n_mlp], dw_r), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_r))
+    flips = np.mean(np.concatenate([(a != b).ravel() for a, b in zip(emu["masks"], ref["masks"])]))
+    print(f"{prec} {name}: emulation dW {e_w:.1e} denc {e_enc:.1e} grid {e_grid:.1e} | "
+          f"FP64 same masks dW {m_w:.1e} grid {m_grid:.1e} | FP64 dW {f_w:.1e} grid {f_grid:.1e} "
+          f"(ReLU flips {flips:.1e})")
+    assert e_w < 2e-4 and e_enc < 2e-4 and e_grid < 2e-4
+    assert m_w < 1e-2 and m_grid < 1e-2
+    assert f_w < FLIP_BOUND[prec] and f_grid < FLIP_BOUND[prec]
+    net.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_train_step_gradient(gpu, oracle, prec):
+    """The train step's loss and gradient (adjoint + backward) against FP64."""
+    import npmg
+    import pyoracle
+
+    cfg = _configs(prec)["small"]
+    cfg["optimizer"] = {"otype": "Adam", "learning_rate": 0.0, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
+                        "l2_reg": 0.0}
+    net = npmg.GuidingNetwork(cfg, seed=5)
+    oc = pyoracle.make_cfg(cfg)
+    n = 4096
+    t = npmg.synth_train(11, n)
+    p = _np(net.get_params(0))
+    loss = net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])
+    assert np.array_equal(_np(net.get_params(0)), p), "lr 0 leaves the parameters"
+    h = {k: _np(v) for k, v in t.items()}
+    # the gradient buffer is internal: rerun the split passes on the same batch
+    raw_gpu = net.forward(t["inputs"])
+    dout_gpu, _ = net.mixture_adjoint(raw_gpu, t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], 128.0)
+    dparams, _ = net.backward(t["inputs"], dout_gpu)
+    g = _np(dparams)
+    emu = _chain(oracle, oc, net, p, h["inputs"], prec)
+    res = {}
+    for key, masks in (("same masks", emu["masks"]), ("own masks", None)):
+        ref = _chain(oracle, oc, net, p, h["inputs"], None, masks=masks)
+        dout, l = oracle.adjoint(oc, ref["raw"], h["tgt_dir"], h["tgt_rgb"], h["tgt_pdf"], 128.0, dtype=np.float64)
+        dw, denc = _backward(ref, dout)
+        gg = oracle.grid_backward(oc, h["inputs"], denc)
+        res[key] = (abs(loss.item() - l.mean()) / abs(l.mean()), _rel2(g[:net.n_mlp], dw), _rel2(g[net.n_mlp:], gg))
+    print(prec, {k: ", ".join(f"{x:.1e}" for x in v) for k, v in res.items()}, "(loss, dW, grid)")
+    e_loss, e_w, e_g = res["same masks"]
+    assert e_loss < 1e-3 and e_w < 1e-2 and e_g < 1e-2
+    f_loss, f_w, f_g = res["own masks"]
+    assert f_loss < 1e-2 and f_w < FLIP_BOUND[prec] and f_g < FLIP_BOUND[prec]
+    net.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_training_converges(gpu, prec):
+    """Training on the analytic field lowers the loss like the FP32 path does."""
+    import npmg
+
+    losses = {}
+    for p in ("fp32", prec):
+        net = npmg.GuidingNetwork(_configs(p)["small"], seed=3)
+        t = npmg.synth_train(21, 1 << 14)
+        ls = []
+        for _ in range(60):
+            ls.append(net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"]).item())
+        losses[p] = ls
+        scale, step = net.train_state()
+        assert step == 60
+        net.close()
+    print(prec, losses[prec][0], losses[prec][-1], "fp32", losses["fp32"][0], losses["fp32"][-1])
+    assert abs(losses[prec][0] - losses["fp32"][0]) / abs(losses["fp32"][0]) < 1e-2
+    assert losses[prec][-1] < 0.9 * losses[prec][0]
+    assert losses[prec][-1] < losses["fp32"][-1] + 0.05 * abs(losses["fp32"][0])
+
+
+def test_dynamic_loss_scale(gpu):
+    import torch
+    import npmg
+
+    cfg = _configs("fp16")["small"]
+    cfg["loss"] = {"scale": 32768.0, "dynamic": True, "scale_window": 3, "grad_clamp": 0.0}
+    net = npmg.GuidingNetwork(cfg, seed=1)
+    t = npmg.synth_train(5, 4096)
+    p0 = net.get_params(0).clone()
+    big = dict(t)
+    big["tgt_rgb"] = t["tgt_rgb"] * 1e6
+    net.train_step(big["inputs"], big["tgt_dir"], big["tgt_rgb"], big["tgt_pdf"])
+    scale, step = net.train_state()
+    assert step == 0 and scale == 16384.0, (scale, step)
+    assert torch.equal(net.get_params(0), p0), "an overflowing step must not change the parameters"
+    for k in range(3):
+        net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])
+    scale, step = net.train_state()
+    assert step == 3 and scale == 32768.0, (scale, step)
+    assert not torch.equal(net.get_params(0), p0)
+    net.close()
+
+
+def test_mfma_train_config_errors(gpu):
+    import npmg
+
+    cfg = npmg.benchmark_config("small", precision="bf16", hidden=6)
+    net = npmg.GuidingNetwork(cfg, seed=1)
+    t = npmg.synth_train(5, 256)
+    with pytest.raises(npmg.NpmgError, match="MFMA training"):
+        net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])
+    net.close()
