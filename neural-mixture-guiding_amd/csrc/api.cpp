@@ -163,8 +163,8 @@ struct npmg_ctx {
     TrainWS ws{};
     float* partials = nullptr;
     size_t partials_cap = 0;
-    void* sort_scratch = nullptr;  // deterministic grid-gradient mode
-    size_t sort_cap = 0;
+    void* scatter_scratch = nullptr;  // grid-gradient binning
+    size_t scatter_cap = 0;
     void* genc = nullptr;  // level-major grid features [L F][n] of the MFMA paths
     uint32_t genc_cap = 0;
     // MFMA training: transposed weight fragments and the operand-type buffers
@@ -181,7 +181,7 @@ struct npmg_ctx {
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
-                        sort_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
+                        scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
                         (void*)mws.raw, (void*)sstate, (void*)ema_pow})
             if (p) (void)hipFree(p);
     }
@@ -272,15 +272,15 @@ struct npmg_ctx {
             uint32_t splits, qps;
             need = wgrad_partials_needed(d, n, &splits, &qps);
         }
-        if (cfg.deterministic_grid) {
-            const size_t need_sort = grid_scatter_sorted_bytes(d, n);
-            if (need_sort > sort_cap) {
-                if (sort_scratch) hip_check(hipFree(sort_scratch), "hipFree");
-                sort_scratch = nullptr;
-                sort_cap = 0;
-                sort_scratch = dalloc<char>(need_sort);
-                sort_cap = need_sort;
-            }
+        const size_t need_scatter = grid_scatter_bytes(d, n, cfg.deterministic_grid);
+        if (need_scatter > scatter_cap) {
+            if (scatter_scratch) hip_check(hipFree(scatter_scratch), "hipFree");
+            scatter_scratch = nullptr;
+            scatter_cap = 0;
+            scatter_scratch = dalloc<char>(need_scatter);
+            scatter_cap = need_scatter;
+            const size_t clear = grid_scatter_clear_bytes(d, cfg.deterministic_grid);
+            if (clear) hip_check(hipMemset(scatter_scratch, 0, clear), "memset");
         }
         if (need > partials_cap) {
             if (partials) hip_check(hipFree(partials), "hipFree");
@@ -328,11 +328,9 @@ struct npmg_ctx {
         if (cfg.n_grid) {
             float* gg = dparams + cfg.n_mlp;
             hip_check(launch_fill(gg, 0.0f, cfg.n_grid, s), "fill");
-            if (cfg.deterministic_grid)
-                hip_check(launch_grid_scatter_sorted(cfg.d, in, ws.dgrid, gg, n, sort_scratch, sort_cap, s),
-                          "grid scatter (sorted)");
-            else
-                hip_check(launch_grid_scatter_atomic(cfg.d, in, ws.dgrid, gg, n, s), "grid scatter");
+            hip_check(launch_grid_scatter(cfg.d, in, ws.dgrid, gg, n, cfg.deterministic_grid, scatter_scratch,
+                                          scatter_cap, s),
+                      "grid scatter");
         }
         if (half())
             hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, partials, partials_cap, dparams, n, of, s),

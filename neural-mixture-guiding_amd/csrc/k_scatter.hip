@@ -1,17 +1,28 @@
 // k_scatter.hip -- grid-gradient scatter, the adjoint of the grid encoding
-// (tiny-cuda-nn kernel_grid_backward, encodings/grid.h:386-509).
+// (tiny-cuda-nn kernel_grid_backward, encodings/grid.h:386-509), without
+// global atomics on the hot path.
 //
-// Atomic mode: grid (ceil(n / 256), n_levels); blocks are dispatched in
-// linear order, so all samples of one level are scattered before the next
-// level starts and the level's gradient slice (at most 2^19 x F floats) stays
-// resident in L2 while it is hit.
-//
-// Deterministic mode: every (sample, level, corner) becomes a pair (global
-// table entry, pair index) plus its F contributions; a stable LSD radix sort
-// (rocPRIM) groups the pairs by entry keeping the pair order (level, sample,
-// corner); each run is then summed in that order by one thread. The result is
-// independent of scheduling.
-#include <rocprim/device/device_radix_sort.hpp>
+// Every (sample, level, corner) contributes w_k * dL/dy_{l,f} to one table
+// entry. The contributions are binned by (level, tile of 8192 / F entries),
+// so that one tile of gradients fits in LDS:
+//   1. count    per (bin, block of 2048 samples), histogram in LDS;
+//   2. scan     per bin over its blocks, then over bins: each (bin, block)
+//               owns a contiguous run of records, in block order; the bins
+//               are cut into chunks of at most kChunk records;
+//   3. place    the records (local word index, F contributions) at their
+//               slots (slot order inside a (bin, block) run is arbitrary);
+//   4. reduce   one workgroup per chunk accumulates its records into the
+//               tile in LDS, then writes the tile: plain stores when the bin
+//               has one chunk (the usual case at fine levels), atomics for
+//               the few coarse bins with several chunks.
+// Atomic mode accumulates in FP32 (LDS float atomics: summation order varies
+// like the reference's atomics). Deterministic mode accumulates exact int64
+// fixed point with a per-level scale 2^S chosen from max |dL/dy| so that no
+// sum can overflow: integer addition is associative, so the result is the
+// same for every schedule; multi-chunk bins combine in an int64 buffer that
+// a final pass converts (and clears for the next call).
+#include <algorithm>
+#include <type_traits>
 
 #include "device.h"
 #include "kernels.h"
@@ -21,123 +32,344 @@ namespace {
 
 using namespace dev;
 
-__global__ void __launch_bounds__(256) k_scatter_atomic(const DevCfg c, const float* __restrict__ in,
-                                                        const float* __restrict__ dgrid, float* __restrict__ grad,
-                                                        uint32_t n) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+constexpr uint32_t kSamplesPerBlock = 2048;
+constexpr uint32_t kTileWords = 8192;  // words of one tile: tile entries x F
+constexpr uint32_t kChunk = 65536;     // records reduced by one workgroup
+constexpr uint32_t kMaxBinsPerLevel = 8192;
+
+struct BinLayout {
+    uint32_t bin_off[33];  // first bin of each level, bin_off[L] = number of bins
+    uint32_t shift;        // log2(tile entries)
+    uint32_t nblk;         // sample blocks
+    uint32_t max_chunks;   // capacity of the chunk table
+};
+
+struct Scratch {
+    uint32_t* counts;      // [bins][nblk] counts, then exclusive offsets within the bin
+    uint32_t* bin_start;   // [bins] first record of the bin
+    uint32_t* bin_chunks;  // [bins]
+    uint4* chunks;         // [max_chunks] (bin, begin, end, chunks of the bin); chunks[-1].x = count
+    uint32_t* maxbits;     // [L] max |dL/dy| of each level (float bits)
+    uint32_t* rec_idx;     // [R] word index within the tile
+    float* rec_val;        // [R][F]
+    unsigned long long* wide;  // [n_grid F] int64 accumulators of multi-chunk bins (deterministic)
+};
+
+__device__ __forceinline__ uint32_t level_of_bin(const BinLayout& b, int L, uint32_t bin) {
+    int l = 0;
+    while (l + 1 < L && bin >= b.bin_off[l + 1]) ++l;
+    return (uint32_t)l;
+}
+
+// Fixed-point scale exponent of level l: |sum| <= n_rec max < 2^62.
+__device__ __forceinline__ int fixed_shift(uint32_t maxbits, uint32_t n_rec) {
+    if (maxbits == 0u) return 0;
+    const int e = (int)((maxbits >> 23) & 0xffu) - 126;  // max < 2^e
+    int r = 0;
+    while (r < 32 && (1ull << r) < (unsigned long long)n_rec) ++r;  // n_rec <= 2^r
+    return 62 - r - e;
+}
+
+__device__ __forceinline__ long long to_fixed(float v, int s) { return __float2ll_rn(ldexpf(v, s)); }
+
+template <bool DET>
+__global__ void __launch_bounds__(256) k_bin_count(const DevCfg c, const BinLayout b, const float* __restrict__ in,
+                                                   const float* __restrict__ dgrid, Scratch sc, uint32_t n) {
+    extern __shared__ uint32_t hist[];
     const int l = blockIdx.y;
-    if (q >= n) return;
-    const int b = c.enc_begin[c.grid_enc];
-    const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
-    uint32_t idx[8];
-    float w[8];
-    const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+    const uint32_t nb = b.bin_off[l + 1] - b.bin_off[l];
+    for (uint32_t i = threadIdx.x; i < nb; i += 256u) hist[i] = 0u;
+    __syncthreads();
+    const int g = c.enc_begin[c.grid_enc];
     const int F = c.F;
-    float* gl = grad + (size_t)c.offsets[l] * F;
-    for (int f = 0; f < F; ++f) {
-        const float d = dgrid[(size_t)(l * F + f) * n + q];
-        if (d == 0.0f) continue;
-        for (int k = 0; k < nc; ++k) atomicAdd(gl + (size_t)idx[k] * F + f, w[k] * d);
+    float mx = 0.0f;
+    for (uint32_t s = 0; s < kSamplesPerBlock; s += 256u) {
+        const uint32_t q = blockIdx.x * kSamplesPerBlock + s + threadIdx.x;
+        if (q >= n) break;
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, in[(size_t)g * n + q], in[(size_t)(g + 1) * n + q],
+                                     in[(size_t)(g + 2) * n + q], idx, w);
+        for (int k = 0; k < nc; ++k) atomicAdd(&hist[idx[k] >> b.shift], 1u);
+        if (DET)
+            for (int f = 0; f < F; ++f) mx = smax(mx, fabsf(dgrid[(size_t)(l * F + f) * n + q]));
     }
+    if (DET) atomicMax(&sc.maxbits[l], __float_as_uint(mx));
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += 256u) sc.counts[(size_t)(b.bin_off[l] + i) * b.nblk + blockIdx.x] = hist[i];
 }
 
-constexpr uint32_t kNoEntry = 0xffffffffu;
+// Block-wide exclusive scan of one value per thread (256 threads); returns the total.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* red, uint32_t& excl) {
+    const uint32_t t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256u; d <<= 1) {
+        const uint32_t o = t >= d ? red[t - d] : 0u;
+        __syncthreads();
+        red[t] += o;
+        __syncthreads();
+    }
+    excl = red[t] - v;
+    const uint32_t total = red[255];
+    __syncthreads();
+    return total;
+}
 
-// Pair p = (l n + q) 8 + k.
-__global__ void __launch_bounds__(256) k_scatter_pairs(const DevCfg c, const float* __restrict__ in,
-                                                       const float* __restrict__ dgrid, uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals, float* __restrict__ contrib,
-                                                       uint32_t n) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+// Exclusive scan of each bin's per-block counts; the bin total goes to bin_start.
+__global__ void __launch_bounds__(256) k_bin_scan(const BinLayout b, Scratch sc) {
+    __shared__ uint32_t red[256];
+    uint32_t* row = sc.counts + (size_t)blockIdx.x * b.nblk;
+    const uint32_t per = (b.nblk + 255u) / 256u;
+    const uint32_t lo = threadIdx.x * per, hi = lo + per < b.nblk ? lo + per : b.nblk;
+    uint32_t sum = 0u;
+    for (uint32_t i = lo; i < hi; ++i) sum += row[i];
+    uint32_t run;
+    const uint32_t total = block_scan(sum, red, run);
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = row[i];
+        row[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) sc.bin_start[blockIdx.x] = total;
+}
+
+// One workgroup: bin totals -> bin starts, chunks per bin, and the chunk table.
+__global__ void __launch_bounds__(256) k_bin_chunks(const BinLayout b, uint32_t n_bins, Scratch sc) {
+    __shared__ uint32_t red[256];
+    const uint32_t per = (n_bins + 255u) / 256u;
+    const uint32_t lo = threadIdx.x * per, hi = lo + per < n_bins ? lo + per : n_bins;
+    uint32_t s_rec = 0u, s_ch = 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t t = sc.bin_start[i];
+        s_rec += t;
+        s_ch += (t + kChunk - 1u) / kChunk;
+    }
+    uint32_t rec, ch;
+    block_scan(s_rec, red, rec);
+    const uint32_t total_ch = block_scan(s_ch, red, ch);
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t t = sc.bin_start[i];
+        const uint32_t k = (t + kChunk - 1u) / kChunk;
+        sc.bin_start[i] = rec;
+        sc.bin_chunks[i] = k;
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t be = rec + j * kChunk;
+            const uint32_t en = be + kChunk < rec + t ? be + kChunk : rec + t;
+            if (ch + j < b.max_chunks) sc.chunks[ch + j] = make_uint4(i, be, en, k);
+        }
+        rec += t;
+        ch += k;
+    }
+    if (threadIdx.x == 0) sc.chunks[-1] = make_uint4(total_ch < b.max_chunks ? total_ch : b.max_chunks, 0u, 0u, 0u);
+}
+
+__global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayout b, const float* __restrict__ in,
+                                                   const float* __restrict__ dgrid, Scratch sc, uint32_t n) {
+    extern __shared__ uint32_t cursor[];
     const int l = blockIdx.y;
-    if (q >= n) return;
-    const int b = c.enc_begin[c.grid_enc];
-    const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
-    uint32_t idx[8];
-    float w[8];
-    const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+    const uint32_t b0 = b.bin_off[l], nb = b.bin_off[l + 1] - b0;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256u)
+        cursor[i] = sc.bin_start[b0 + i] + sc.counts[(size_t)(b0 + i) * b.nblk + blockIdx.x];
+    __syncthreads();
+    const int g = c.enc_begin[c.grid_enc];
     const int F = c.F;
-    const size_t p0 = ((size_t)l * n + q) * 8;
-    for (int k = 0; k < 8; ++k) {
-        keys[p0 + k] = k < nc ? c.offsets[l] + idx[k] : kNoEntry;
-        vals[p0 + k] = (uint32_t)(p0 + k);
-        for (int f = 0; f < F; ++f)
-            contrib[(p0 + k) * F + f] = k < nc ? w[k] * dgrid[(size_t)(l * F + f) * n + q] : 0.0f;
+    const uint32_t mask = (1u << b.shift) - 1u;
+    for (uint32_t s = 0; s < kSamplesPerBlock; s += 256u) {
+        const uint32_t q = blockIdx.x * kSamplesPerBlock + s + threadIdx.x;
+        if (q >= n) break;
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, in[(size_t)g * n + q], in[(size_t)(g + 1) * n + q],
+                                     in[(size_t)(g + 2) * n + q], idx, w);
+        if (F == 2) {
+            const float d0 = dgrid[(size_t)(2 * l) * n + q], d1 = dgrid[(size_t)(2 * l + 1) * n + q];
+            for (int k = 0; k < nc; ++k) {
+                const uint32_t slot = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
+                sc.rec_idx[slot] = (idx[k] & mask) * 2u;
+                reinterpret_cast<float2*>(sc.rec_val)[slot] = make_float2(w[k] * d0, w[k] * d1);
+            }
+        } else {
+            for (int k = 0; k < nc; ++k) {
+                const uint32_t slot = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
+                sc.rec_idx[slot] = (idx[k] & mask) * (uint32_t)F;
+                for (int f = 0; f < F; ++f)
+                    sc.rec_val[(size_t)slot * F + f] = w[k] * dgrid[(size_t)(l * F + f) * n + q];
+            }
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) k_scatter_runs(const DevCfg c, const uint32_t* __restrict__ keys,
-                                                      const uint32_t* __restrict__ vals,
-                                                      const float* __restrict__ contrib, float* __restrict__ grad,
-                                                      size_t n_pairs) {
-    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
-    if (i >= n_pairs) return;
-    const uint32_t key = keys[i];
-    if (key == kNoEntry || (i > 0 && keys[i - 1] == key)) return;
+template <bool DET>
+__global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLayout b, Scratch sc,
+                                                    float* __restrict__ grad, uint32_t n) {
+    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
+    __shared__ Acc tile[kTileWords];
+    if (blockIdx.x >= sc.chunks[-1].x) return;
+    const uint4 ch = sc.chunks[blockIdx.x];
     const int F = c.F;
-    float acc[8] = {};
-    for (size_t j = i; j < n_pairs && keys[j] == key; ++j) {
-        const size_t p = vals[j];
-        for (int f = 0; f < F; ++f) acc[f] += contrib[p * F + f];
+    const uint32_t l = level_of_bin(b, c.n_levels, ch.x);
+    const uint32_t t = ch.x - b.bin_off[l];
+    const uint32_t size = c.offsets[l + 1] - c.offsets[l];
+    const uint32_t e0 = t << b.shift;
+    const uint32_t entries = size - e0 < (1u << b.shift) ? size - e0 : (1u << b.shift);
+    const uint32_t words = entries * (uint32_t)F;
+    const int s = DET ? fixed_shift(sc.maxbits[l], n * 8u) : 0;
+    for (uint32_t i = threadIdx.x; i < kTileWords; i += 256u) tile[i] = (Acc)0;
+    __syncthreads();
+    for (uint32_t r = ch.y + threadIdx.x; r < ch.z; r += 256u) {
+        const uint32_t wi = sc.rec_idx[r];
+        if (F == 2) {
+            const float2 v = reinterpret_cast<const float2*>(sc.rec_val)[r];
+            if (DET) {
+                atomicAdd(&tile[wi], (Acc)to_fixed(v.x, s));
+                atomicAdd(&tile[wi + 1], (Acc)to_fixed(v.y, s));
+            } else {
+                atomicAdd(&tile[wi], v.x);
+                atomicAdd(&tile[wi + 1], v.y);
+            }
+        } else {
+            for (int f = 0; f < F; ++f) {
+                const float v = sc.rec_val[(size_t)r * F + f];
+                if (DET) atomicAdd(&tile[wi + f], (Acc)to_fixed(v, s));
+                else atomicAdd(&tile[wi + f], v);
+            }
+        }
     }
-    for (int f = 0; f < F; ++f) grad[(size_t)key * F + f] = acc[f];
+    __syncthreads();
+    const size_t gbase = ((size_t)c.offsets[l] + e0) * F;
+    if (ch.w == 1u) {
+        for (uint32_t i = threadIdx.x; i < words; i += 256u) {
+            if (DET) grad[gbase + i] = ldexpf((float)(long long)tile[i], -s);
+            else grad[gbase + i] = (float)tile[i];
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < words; i += 256u) {
+            if (tile[i] == (Acc)0) continue;
+            if (DET) atomicAdd(&sc.wide[gbase + i], (unsigned long long)tile[i]);
+            else atomicAdd(&grad[gbase + i], (float)tile[i]);
+        }
+    }
 }
 
-int key_bits(const DevCfg& c) {
-    const uint32_t total = c.offsets[c.n_levels];
-    int bits = 1;
-    while (bits < 32 && (1ull << bits) <= (unsigned long long)total) ++bits;
-    return bits;
+// Deterministic mode: multi-chunk bins, int64 -> float, and clear for the next call.
+__global__ void __launch_bounds__(256) k_bin_finish(const DevCfg c, const BinLayout b, Scratch sc,
+                                                    float* __restrict__ grad, uint32_t n) {
+    const uint32_t bin = blockIdx.x;
+    if (sc.bin_chunks[bin] < 2u) return;
+    const int F = c.F;
+    const uint32_t l = level_of_bin(b, c.n_levels, bin);
+    const uint32_t t = bin - b.bin_off[l];
+    const uint32_t size = c.offsets[l + 1] - c.offsets[l];
+    const uint32_t e0 = t << b.shift;
+    const uint32_t entries = size - e0 < (1u << b.shift) ? size - e0 : (1u << b.shift);
+    const int s = fixed_shift(sc.maxbits[l], n * 8u);
+    const size_t gbase = ((size_t)c.offsets[l] + e0) * F;
+    for (uint32_t i = threadIdx.x; i < entries * (uint32_t)F; i += 256u) {
+        grad[gbase + i] = ldexpf((float)(long long)sc.wide[gbase + i], -s);
+        sc.wide[gbase + i] = 0ull;
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+BinLayout make_layout(const DevCfg& c, uint32_t n, uint32_t* n_bins) {
+    BinLayout b{};
+    int fl = 0;
+    while ((1 << (fl + 1)) <= c.F) ++fl;
+    b.shift = 13u - (uint32_t)fl;  // (8192 / F) entries per tile
+    uint32_t bins = 0;
+    for (int l = 0; l < c.n_levels; ++l) {
+        b.bin_off[l] = bins;
+        const uint32_t size = c.offsets[l + 1] - c.offsets[l];
+        bins += (size + (1u << b.shift) - 1u) >> b.shift;
+    }
+    b.bin_off[c.n_levels] = bins;
+    b.nblk = (n + kSamplesPerBlock - 1u) / kSamplesPerBlock;
+    const size_t recs = (size_t)n * c.n_levels * 8u;
+    b.max_chunks = (uint32_t)(recs / kChunk + bins + 1u);
+    *n_bins = bins;
+    return b;
+}
+
+struct Carve {
+    char* p;
+    size_t used = 0;
+    template <class T>
+    T* take(size_t count) {
+        T* r = reinterpret_cast<T*>(p ? p + used : nullptr);
+        used += (count * sizeof(T) + 255u) / 256u * 256u;
+        return r;
+    }
+};
+
+Scratch carve(const DevCfg& c, const BinLayout& b, uint32_t n_bins, uint32_t n, bool det, void* base, size_t* bytes) {
+    Carve cv{static_cast<char*>(base)};
+    Scratch sc{};
+    const size_t recs = (size_t)n * c.n_levels * 8u;
+    // int64 accumulators first: they are cleared once at allocation and kept clear.
+    sc.wide = det ? cv.take<unsigned long long>((size_t)c.offsets[c.n_levels] * c.F) : nullptr;
+    sc.counts = cv.take<uint32_t>((size_t)n_bins * b.nblk);
+    sc.bin_start = cv.take<uint32_t>(n_bins);
+    sc.bin_chunks = cv.take<uint32_t>(n_bins);
+    uint4* ct = cv.take<uint4>((size_t)b.max_chunks + 1u);
+    sc.chunks = ct ? ct + 1 : nullptr;
+    sc.maxbits = cv.take<uint32_t>(32);
+    sc.rec_idx = cv.take<uint32_t>(recs);
+    sc.rec_val = cv.take<float>(recs * c.F);
+    *bytes = cv.used;
+    return sc;
 }
 
 }  // namespace
 
-hipError_t launch_grid_scatter_atomic(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
-                                      hipStream_t s) {
-    if (n == 0 || c.grid_enc < 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_atomic, dim3((n + 255) / 256, c.n_levels), dim3(256), 0, s, c, in, dgrid, grad, n);
-    return hipGetLastError();
+size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic) {
+    if (c.grid_enc < 0 || n == 0) return 0;
+    uint32_t n_bins;
+    const BinLayout b = make_layout(c, n, &n_bins);
+    size_t bytes = 0;
+    carve(c, b, n_bins, n, deterministic, nullptr, &bytes);
+    return bytes;
 }
 
-// Scratch layout: keys_in, keys_out, vals_in, vals_out (n_pairs u32 each),
-// contributions (n_pairs F floats), then the sort's temporary storage.
-size_t grid_scatter_sorted_bytes(const DevCfg& c, uint32_t n) {
-    if (c.grid_enc < 0) return 0;
-    const size_t np = (size_t)n * c.n_levels * 8;
-    size_t tmp = 0;
-    (void)rocprim::radix_sort_pairs((void*)nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                              (uint32_t*)nullptr, np, 0, key_bits(c) + 1, (hipStream_t)0);
-    return np * 4 * 4 + np * c.F * 4 + ((tmp + 255) / 256) * 256 + 1024;
+size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic) {
+    if (c.grid_enc < 0 || !deterministic) return 0;
+    return ((size_t)c.offsets[c.n_levels] * c.F * 8u + 255u) / 256u * 256u;
 }
 
-hipError_t launch_grid_scatter_sorted(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
-                                      void* scratch, size_t scratch_bytes, hipStream_t s) {
+hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
+                               bool deterministic, void* scratch, size_t scratch_bytes, hipStream_t s) {
     if (n == 0 || c.grid_enc < 0) return hipSuccess;
-    const size_t np = (size_t)n * c.n_levels * 8;
-    if (np >= 0xffffffffull) return hipErrorInvalidValue;
-    if (scratch_bytes < grid_scatter_sorted_bytes(c, n)) return hipErrorInvalidValue;
-    char* p = static_cast<char*>(scratch);
-    auto carve = [&](size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
-    uint32_t* k_in = reinterpret_cast<uint32_t*>(carve(np * 4));
-    uint32_t* k_out = reinterpret_cast<uint32_t*>(carve(np * 4));
-    uint32_t* v_in = reinterpret_cast<uint32_t*>(carve(np * 4));
-    uint32_t* v_out = reinterpret_cast<uint32_t*>(carve(np * 4));
-    float* contrib = reinterpret_cast<float*>(carve(np * c.F * 4));
-    size_t tmp = 0;
-    // bit key_bits(c) distinguishes the sentinel 0xffffffff from every entry
-    const unsigned end_bit = (unsigned)key_bits(c) + 1;
-    hipError_t e = rocprim::radix_sort_pairs((void*)nullptr, tmp, k_in, k_out, v_in, v_out, np, 0u, end_bit, s);
-    if (e != hipSuccess) return e;
-    void* sort_tmp = carve(tmp);
-    if ((size_t)(p - static_cast<char*>(scratch)) > scratch_bytes) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scatter_pairs, dim3((n + 255) / 256, c.n_levels), dim3(256), 0, s, c, in, dgrid, k_in, v_in,
-                       contrib, n);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = rocprim::radix_sort_pairs(sort_tmp, tmp, k_in, k_out, v_in, v_out, np, 0u, end_bit, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter_runs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, c, k_out, v_out, contrib,
-                       grad, np);
+    if ((size_t)n * c.n_levels * 8u >= 0xffffffffull || c.F > 8) return hipErrorInvalidValue;
+    uint32_t n_bins;
+    const BinLayout b = make_layout(c, n, &n_bins);
+    for (int l = 0; l < c.n_levels; ++l)
+        if (b.bin_off[l + 1] - b.bin_off[l] > kMaxBinsPerLevel) return hipErrorInvalidValue;
+    size_t need = 0;
+    const Scratch sc = carve(c, b, n_bins, n, deterministic, scratch, &need);
+    if (!scratch || need > scratch_bytes) return hipErrorInvalidValue;
+    uint32_t bmax = 0;
+    for (int l = 0; l < c.n_levels; ++l) bmax = std::max(bmax, b.bin_off[l + 1] - b.bin_off[l]);
+    const size_t hist_lds = (size_t)bmax * 4u;
+    const dim3 grid(b.nblk, c.n_levels);
+    if (deterministic) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, sc.maxbits, 32u);
+        hipLaunchKernelGGL(k_bin_count<true>, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
+    } else {
+        hipLaunchKernelGGL(k_bin_count<false>, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
+    }
+    hipLaunchKernelGGL(k_bin_scan, dim3(n_bins), dim3(256), 0, s, b, sc);
+    hipLaunchKernelGGL(k_bin_chunks, dim3(1), dim3(256), 0, s, b, n_bins, sc);
+    hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
+    if (deterministic) {
+        hipLaunchKernelGGL(k_bin_reduce<true>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
+        hipLaunchKernelGGL(k_bin_finish, dim3(n_bins), dim3(256), 0, s, c, b, sc, grad, n);
+    } else {
+        hipLaunchKernelGGL(k_bin_reduce<false>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
+    }
     return hipGetLastError();
 }
 

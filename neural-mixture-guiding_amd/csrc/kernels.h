@@ -78,15 +78,15 @@ hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const flo
 // features) into dgrid [L F][n], dL/denc into denc (optional).
 hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
                                       uint32_t n, int n_cu, hipStream_t s);
-// Grid-gradient scatter (tiny-cuda-nn kernel_grid_backward, grid.h:386-509):
-// atomic mode, one level per grid row so each level's slice stays in L2.
-hipError_t launch_grid_scatter_atomic(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
-                                      uint32_t n, hipStream_t s);
-// Deterministic mode: (table entry, contribution) pairs, stable radix sort by
-// entry, in-order sum of each run. ws_bytes reports/uses the scratch size.
-size_t grid_scatter_sorted_bytes(const DevCfg& c, uint32_t n);
-hipError_t launch_grid_scatter_sorted(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
-                                      uint32_t n, void* scratch, size_t scratch_bytes, hipStream_t s);
+// Grid-gradient scatter (tiny-cuda-nn kernel_grid_backward, grid.h:386-509),
+// binned by table tile and reduced in LDS; grid_grad must be zero on entry.
+// Deterministic mode sums exactly in int64 fixed point. Scratch: at least
+// grid_scatter_bytes; its first grid_scatter_clear_bytes must be zero before
+// the first call (the scatter leaves them zero again).
+size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic);
+size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic);
+hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad, uint32_t n,
+                               bool deterministic, void* scratch, size_t scratch_bytes, hipStream_t s);
 // Weight gradients dW_l = sum_q g_l x_l^T, deterministic split-K reduction.
 // Non-finite sums raise *overflow when it is given.
 hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,

@@ -1,4 +1,9 @@
-"""Grid-gradient scatter modes (tiny-cuda-nn kernel_grid_backward): atomic and deterministic (sorted)."""
+"""Grid-gradient scatter (tiny-cuda-nn kernel_grid_backward), binned by table
+tile and reduced in LDS: atomic mode (FP32 sums, order varies) and
+deterministic mode (exact int64 fixed-point sums). Both against the oracle's
+FP64 grid backward on the library's own dL/denc. The small case has coarse
+bins that span several reduction chunks; the full case is the benchmark
+shape (16 levels x 2^19 entries, 2^18 samples)."""
 import numpy as np
 import pytest
 
@@ -9,17 +14,17 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.mark.parametrize("size", ["small", "reference"])
+@pytest.mark.parametrize("size", ["small", "reference", "full"])
 def test_deterministic_mode_bitwise_repeatable_and_matches_fp64(gpu, oracle, size):
     import torch
     import npmg
     import pyoracle
 
-    base = npmg.benchmark_config("small") if size == "small" else npmg.reference_config()
+    base = npmg.reference_config() if size == "reference" else npmg.benchmark_config(size)
     cfg_det = dict(base, grid_gradient="deterministic")
     net_d = npmg.GuidingNetwork(cfg_det, seed=3)
     net_a = npmg.GuidingNetwork(dict(base, grid_gradient="atomic"), seed=3)
-    n = 20000
+    n = (1 << 18) if size == "full" else 20000
     q = npmg.synth_queries(12, n)
     dout = torch.from_numpy(np.random.default_rng(4).normal(size=(net_d.padded_output, n)).astype(np.float32)).cuda()
     g1, _ = net_d.backward(q["inputs"], dout)
