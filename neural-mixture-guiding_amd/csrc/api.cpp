@@ -172,6 +172,7 @@ struct npmg_ctx {
     TrainMfmaWS mws{};
     // loss scale, overflow flag and step count on the device; decay^t table of the EMA debiasing
     ScaleState* sstate = nullptr;
+    float* mean_partials = nullptr;  // [256]
     float* ema_pow = nullptr;
     uint32_t ema_pow_len = 0;
 
@@ -182,7 +183,7 @@ struct npmg_ctx {
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
-                        (void*)mws.raw, (void*)sstate, (void*)ema_pow})
+                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials})
             if (p) (void)hipFree(p);
     }
 
@@ -208,15 +209,20 @@ struct npmg_ctx {
         genc_cap = n;
     }
 
-    void refresh_layouts(hipStream_t s) {
+    // Kernel layouts of the parameters: transposed FP32 weights, MFMA fragments and,
+    // unless the optimiser already wrote them, the operand-type grid tables.
+    void refresh_layouts(hipStream_t s, bool tables = true) {
         hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
         hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
         if (half()) {
             hip_check(launch_pack_frags(cfg.d, cfg.precision, master, frag_train, s), "pack");
             if (bfrag_train) hip_check(launch_pack_bfrags(cfg.d, cfg.precision, master, bfrag_train, s), "pack");
             hip_check(launch_pack_frags(cfg.d, cfg.precision, infer_params(), frag_infer, s), "pack");
-            hip_check(launch_to_half(cfg.precision, master + cfg.n_mlp, htab_train, cfg.n_grid, s), "table");
-            hip_check(launch_to_half(cfg.precision, infer_params() + cfg.n_mlp, htab_infer, cfg.n_grid, s), "table");
+            if (tables) {
+                hip_check(launch_to_half(cfg.precision, master + cfg.n_mlp, htab_train, cfg.n_grid, s), "table");
+                hip_check(launch_to_half(cfg.precision, infer_params() + cfg.n_mlp, htab_infer, cfg.n_grid, s),
+                          "table");
+            }
         }
     }
 
@@ -292,7 +298,7 @@ struct npmg_ctx {
         ws_cap = n;
     }
 
-    LossScale state_scale(uint32_t n) const { return LossScale{&sstate->scale, 1.0f / (float)n}; }
+    LossScale state_scale(uint32_t n) const { return LossScale{&sstate->scale, 0.0f, 1.0f / (float)n}; }
 
     // Forward with the training weights keeping what the backward pass needs;
     // with targets, the loss and dL/draw (scaled by ls) go to ws.loss / ws.dout.
@@ -358,9 +364,12 @@ struct npmg_ctx {
         a.l2_reg = cfg.l2_reg;
         a.ema = ema != nullptr;
         a.ema_decay = cfg.ema_decay;
+        a.half_precision = half() ? cfg.precision : PREC_FP32;
+        a.half_train = htab_train;
+        a.half_infer = htab_infer;
         hip_check(launch_adam(a, master, dparams, m, v, steps, ema, s), "adam");
         hip_check(launch_scale_update(sstate, a.check_overflow, cfg.scale_window, kMaxLossScale, s), "loss scale");
-        refresh_layouts(s);
+        refresh_layouts(s, false);
     }
 };
 
@@ -433,6 +442,7 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
             if (mfma_train_supported(c.d)) ctx->bfrag_train = dalloc<uint16_t>(bfrag_elements(c.d));
         }
         ctx->sstate = dalloc<ScaleState>(1);
+        ctx->mean_partials = dalloc<float>(256);
         const ScaleState st0{c.loss_scale, 0u, 0u, 0u};
         hip_check(hipMemcpy(ctx->sstate, &st0, sizeof(st0), hipMemcpyHostToDevice), "upload");
         // decay^t for the EMA debiasing (tcnn ema.h:110-111 evaluates std::pow in
@@ -621,7 +631,7 @@ npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* t
     return guarded(ctx, [&] {
         need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && dout && loss), "null buffer");
         hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf,
-                                 LossScale{nullptr, n ? loss_scale / (float)n : 0.0f}, dout, loss, n, S(stream)),
+                                 LossScale{nullptr, loss_scale, n ? 1.0f / (float)n : 0.0f}, dout, loss, n, S(stream)),
                   "adjoint");
     });
 }
@@ -632,7 +642,7 @@ npmg_status npmg_backward(npmg_ctx* ctx, const float* in, const float* dout, uin
     return guarded(ctx, [&] {
         ctx->check_train();
         need(n > 0 && in && dout && dparams, "null buffer or empty batch");
-        ctx->train_forward(in, nullptr, nullptr, nullptr, n, LossScale{nullptr, 1.0f}, S(stream));
+        ctx->train_forward(in, nullptr, nullptr, nullptr, n, LossScale{nullptr, 1.0f, 1.0f / (float)n}, S(stream));
         ctx->train_backward(in, dout, dparams, denc, n, false, S(stream));
     });
 }
@@ -656,7 +666,7 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
         ctx->train_forward(in, tgt_dir, tgt_rgb, tgt_pdf, n, ctx->state_scale(n), s);
         ctx->train_backward(in, ctx->ws.dout, ctx->grad, nullptr, n, ctx->cfg.dynamic_scale, s);
         ctx->optimizer_step(ctx->grad, 0.0f, s);
-        if (loss_out) hip_check(launch_mean(ctx->ws.loss, n, loss_out, s), "loss mean");
+        if (loss_out) hip_check(launch_mean(ctx->ws.loss, n, ctx->mean_partials, loss_out, s), "loss mean");
     });
 }
 

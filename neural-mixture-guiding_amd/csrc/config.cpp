@@ -217,10 +217,13 @@ Config parse_config(const std::string& text) {
         c.dynamic_scale = loss.boolean("dynamic", false);
         c.scale_window = (uint32_t)loss.number("scale_window", 1000);
         d.grad_clamp = (float)loss.number("grad_clamp", 150.0);
+        d.clamp_scale = (float)loss.number("clamp_scale", 128.0);
     } else {
         d.grad_clamp = 150.0f;
+        d.clamp_scale = 128.0f;
     }
     if (!(c.loss_scale > 0)) bad("loss scale must be positive");
+    if (!(d.clamp_scale > 0)) bad("loss clamp_scale must be positive");
 
     const std::string prec = root.string("precision", "fp32");
     if (ieq(prec, "fp32")) c.precision = PREC_FP32;

@@ -38,6 +38,7 @@ struct DevCfg {
     uint32_t layer_off[kMaxLayers];  // offset of each weight matrix in the parameter vector
     int K, mapping, numerics, mixture;
     float cos_kappa, grad_clamp;
+    float clamp_scale;  // loss scale at which grad_clamp applies (threshold grad_clamp * scale / clamp_scale)
 };
 
 struct Config {

@@ -477,10 +477,12 @@ __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
 // KL loss -f/q log(p + eps) and its adjoint with respect to the raw outputs
 // (train.cu:69-124 with gradients_probability, distribution.h:143-181).
 // dp/dlambda_s is evaluated as (v_s - p) / Lambda, the closed form of the
-// reference's O(K^2) expression. `ls_over_n` is loss_scale / batch size.
+// reference's O(K^2) expression. `ls_over_n` is loss_scale / batch size;
+// clamp_hi the upper clamp of the scaled gradient (0: none).
 template <int K, class Raw, class Out>
 __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
-                                                 float b, float wipdf, float ls_over_n, Out&& out) {
+                                                 float b, float wipdf, float ls_over_n, float clamp_hi,
+                                                 Out&& out) {
     const bool ref = c.numerics == NUM_REFERENCE;
     Mixture<K> m;
     build_mixture<K>(c, x, m);
@@ -497,10 +499,9 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
     const float guide = p + kEps;
     const float prefix = -Li / wiPdf / guide * ls_over_n;
     const float loss = -Li / wiPdf * fm::log(guide);
-    const float clampv_hi = c.grad_clamp;
     auto finish = [&](int j, float v) {
         if (!fm::isfinite_(v)) v = 0.0f;
-        if (clampv_hi > 0.0f) v = smin(clampv_hi, v);
+        if (clamp_hi > 0.0f) v = smin(clamp_hi, v);
         out(j, v);
     };
 #pragma unroll

@@ -9,20 +9,25 @@
 //   2. scan     per bin over its blocks, then over bins: each (bin, block)
 //               owns a contiguous run of records, in block order; the bins
 //               are cut into chunks of at most kChunk records;
-//   3. place    the records (local word index, F contributions) at their
-//               slots (slot order inside a (bin, block) run is arbitrary);
+//   3. place    the records (local word index, F contributions; 1 + F
+//               words) at their slots (slot order inside a (bin, block) run
+//               is arbitrary);
 //   4. reduce   one workgroup per chunk accumulates its records into the
 //               tile in LDS, then writes the tile: plain stores when the bin
 //               has one chunk (the usual case at fine levels), atomics for
 //               the few coarse bins with several chunks.
-// Atomic mode accumulates in FP32 (LDS float atomics: summation order varies
-// like the reference's atomics). Deterministic mode accumulates exact int64
-// fixed point with a per-level scale 2^S chosen from max |dL/dy| so that no
-// sum can overflow: integer addition is associative, so the result is the
-// same for every schedule; multi-chunk bins combine in an int64 buffer that
-// a final pass converts (and clears for the next call).
+// The LDS accumulators are int64 fixed point with a per-level scale 2^S chosen
+// from max |dL/dy| so that no sum can overflow (|sum| <= 8 n max < 2^62): the
+// sums are exact, so the order of the LDS atomics does not matter, and the
+// resolution is 2^-(62 - log2(8 n)) max |dL/dy| (2^-41 of the level's largest
+// contribution at n = 2^18). Integer LDS atomics are also several times faster
+// than FP32 ones on gfx950. The modes differ only for the coarse bins that
+// span several chunks: atomic mode adds each chunk's converted tile to the
+// gradient with FP32 global atomics (order-dependent rounding, like the
+// reference's atomics); deterministic mode adds the int64 tiles into an int64
+// buffer that a final pass converts (and clears for the next call), so every
+// schedule gives the same bits.
 #include <algorithm>
-#include <type_traits>
 
 #include "device.h"
 #include "kernels.h"
@@ -50,8 +55,7 @@ struct Scratch {
     uint32_t* bin_chunks;  // [bins]
     uint4* chunks;         // [max_chunks] (bin, begin, end, chunks of the bin); chunks[-1].x = count
     uint32_t* maxbits;     // [L] max |dL/dy| of each level (float bits)
-    uint32_t* rec_idx;     // [R] word index within the tile
-    float* rec_val;        // [R][F]
+    uint32_t* rec;         // [R][1 + F] records: word index within the tile, then F contributions
     unsigned long long* wide;  // [n_grid F] int64 accumulators of multi-chunk bins (deterministic)
 };
 
@@ -72,10 +76,26 @@ __device__ __forceinline__ int fixed_shift(uint32_t maxbits, uint32_t n_rec) {
 
 __device__ __forceinline__ long long to_fixed(float v, int s) { return __float2ll_rn(ldexpf(v, s)); }
 
-template <bool DET>
+constexpr uint32_t kPerThread = kSamplesPerBlock / 256u;
+
+// Positions of the thread's samples (sample i: block base + 256 i + thread), all
+// loads issued before any use; samples past n read sample n - 1.
+__device__ __forceinline__ void load_positions(const float* __restrict__ in, int g, uint32_t n, float* px, float* py,
+                                               float* pz) {
+#pragma unroll
+    for (int i = 0; i < (int)kPerThread; ++i) {
+        uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
+        q = q < n ? q : n - 1u;
+        px[i] = in[(size_t)g * n + q];
+        py[i] = in[(size_t)(g + 1) * n + q];
+        pz[i] = in[(size_t)(g + 2) * n + q];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bin_count(const DevCfg c, const BinLayout b, const float* __restrict__ in,
                                                    const float* __restrict__ dgrid, Scratch sc, uint32_t n) {
     extern __shared__ uint32_t hist[];
+    __shared__ float wave_max[4];
     const int l = blockIdx.y;
     const uint32_t nb = b.bin_off[l + 1] - b.bin_off[l];
     for (uint32_t i = threadIdx.x; i < nb; i += 256u) hist[i] = 0u;
@@ -83,19 +103,25 @@ __global__ void __launch_bounds__(256) k_bin_count(const DevCfg c, const BinLayo
     const int g = c.enc_begin[c.grid_enc];
     const int F = c.F;
     float mx = 0.0f;
-    for (uint32_t s = 0; s < kSamplesPerBlock; s += 256u) {
-        const uint32_t q = blockIdx.x * kSamplesPerBlock + s + threadIdx.x;
+    float px[kPerThread], py[kPerThread], pz[kPerThread];
+    load_positions(in, g, n, px, py, pz);
+#pragma unroll
+    for (int i = 0; i < (int)kPerThread; ++i) {
+        const uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
         if (q >= n) break;
         uint32_t idx[8];
         float w[8];
-        const int nc = level_corners(c, l, in[(size_t)g * n + q], in[(size_t)(g + 1) * n + q],
-                                     in[(size_t)(g + 2) * n + q], idx, w);
+        const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
         for (int k = 0; k < nc; ++k) atomicAdd(&hist[idx[k] >> b.shift], 1u);
-        if (DET)
-            for (int f = 0; f < F; ++f) mx = smax(mx, fabsf(dgrid[(size_t)(l * F + f) * n + q]));
+        for (int f = 0; f < F; ++f) mx = smax(mx, fabsf(dgrid[(size_t)(l * F + f) * n + q]));
     }
-    if (DET) atomicMax(&sc.maxbits[l], __float_as_uint(mx));
+    // max |dL/dy| of the block, then one global atomic per block (bits of
+    // non-negative floats order like the values)
+    for (int o = 32; o > 0; o >>= 1) mx = smax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63u) == 0) wave_max[threadIdx.x >> 6] = mx;
     __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(&sc.maxbits[l], __float_as_uint(smax(smax(wave_max[0], wave_max[1]), smax(wave_max[2], wave_max[3]))));
     for (uint32_t i = threadIdx.x; i < nb; i += 256u) sc.counts[(size_t)(b.bin_off[l] + i) * b.nblk + blockIdx.x] = hist[i];
 }
 
@@ -175,26 +201,40 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
     const int g = c.enc_begin[c.grid_enc];
     const int F = c.F;
     const uint32_t mask = (1u << b.shift) - 1u;
-    for (uint32_t s = 0; s < kSamplesPerBlock; s += 256u) {
-        const uint32_t q = blockIdx.x * kSamplesPerBlock + s + threadIdx.x;
+    float px[kPerThread], py[kPerThread], pz[kPerThread];
+    load_positions(in, g, n, px, py, pz);
+#pragma unroll
+    for (int i = 0; i < (int)kPerThread; ++i) {
+        const uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
         if (q >= n) break;
-        uint32_t idx[8];
+        uint32_t idx[8], slot[8];
         float w[8];
-        const int nc = level_corners(c, l, in[(size_t)g * n + q], in[(size_t)(g + 1) * n + q],
-                                     in[(size_t)(g + 2) * n + q], idx, w);
+        const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
         if (F == 2) {
             const float d0 = dgrid[(size_t)(2 * l) * n + q], d1 = dgrid[(size_t)(2 * l + 1) * n + q];
-            for (int k = 0; k < nc; ++k) {
-                const uint32_t slot = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
-                sc.rec_idx[slot] = (idx[k] & mask) * 2u;
-                reinterpret_cast<float2*>(sc.rec_val)[slot] = make_float2(w[k] * d0, w[k] * d1);
+            if (nc == 8) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) slot[k] = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    uint32_t* r = sc.rec + (size_t)slot[k] * 3u;
+                    r[0] = (idx[k] & mask) * 2u;
+                    r[1] = __float_as_uint(w[k] * d0);
+                    r[2] = __float_as_uint(w[k] * d1);
+                }
+            } else {
+                for (int k = 0; k < nc; ++k) {
+                    uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * 3u;
+                    r[0] = (idx[k] & mask) * 2u;
+                    r[1] = __float_as_uint(w[k] * d0);
+                    r[2] = __float_as_uint(w[k] * d1);
+                }
             }
         } else {
             for (int k = 0; k < nc; ++k) {
-                const uint32_t slot = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
-                sc.rec_idx[slot] = (idx[k] & mask) * (uint32_t)F;
-                for (int f = 0; f < F; ++f)
-                    sc.rec_val[(size_t)slot * F + f] = w[k] * dgrid[(size_t)(l * F + f) * n + q];
+                uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * (1u + F);
+                r[0] = (idx[k] & mask) * (uint32_t)F;
+                for (int f = 0; f < F; ++f) r[1 + f] = __float_as_uint(w[k] * dgrid[(size_t)(l * F + f) * n + q]);
             }
         }
     }
@@ -203,7 +243,7 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
 template <bool DET>
 __global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLayout b, Scratch sc,
                                                     float* __restrict__ grad, uint32_t n) {
-    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
+    using Acc = unsigned long long;
     __shared__ Acc tile[kTileWords];
     if (blockIdx.x >= sc.chunks[-1].x) return;
     const uint4 ch = sc.chunks[blockIdx.x];
@@ -214,40 +254,50 @@ __global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLay
     const uint32_t e0 = t << b.shift;
     const uint32_t entries = size - e0 < (1u << b.shift) ? size - e0 : (1u << b.shift);
     const uint32_t words = entries * (uint32_t)F;
-    const int s = DET ? fixed_shift(sc.maxbits[l], n * 8u) : 0;
+    const int s = fixed_shift(sc.maxbits[l], n * 8u);
     for (uint32_t i = threadIdx.x; i < kTileWords; i += 256u) tile[i] = (Acc)0;
     __syncthreads();
-    for (uint32_t r = ch.y + threadIdx.x; r < ch.z; r += 256u) {
-        const uint32_t wi = sc.rec_idx[r];
-        if (F == 2) {
-            const float2 v = reinterpret_cast<const float2*>(sc.rec_val)[r];
-            if (DET) {
-                atomicAdd(&tile[wi], (Acc)to_fixed(v.x, s));
-                atomicAdd(&tile[wi + 1], (Acc)to_fixed(v.y, s));
-            } else {
-                atomicAdd(&tile[wi], v.x);
-                atomicAdd(&tile[wi + 1], v.y);
+    auto add = [&](uint32_t wi, float v) { atomicAdd(&tile[wi], (Acc)to_fixed(v, s)); };
+    if (F == 2) {
+        // 8 records per thread in flight: all loads, then the LDS atomics.
+        constexpr uint32_t U = 8;
+        uint32_t r0 = ch.y + threadIdx.x;
+        for (; r0 + 256u * (U - 1u) < ch.z; r0 += 256u * U) {
+            uint32_t wi[U];
+            float v0[U], v1[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t* rp = sc.rec + (size_t)(r0 + 256u * u) * 3u;
+                wi[u] = rp[0];
+                v0[u] = __uint_as_float(rp[1]);
+                v1[u] = __uint_as_float(rp[2]);
             }
-        } else {
-            for (int f = 0; f < F; ++f) {
-                const float v = sc.rec_val[(size_t)r * F + f];
-                if (DET) atomicAdd(&tile[wi + f], (Acc)to_fixed(v, s));
-                else atomicAdd(&tile[wi + f], v);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                add(wi[u], v0[u]);
+                add(wi[u] + 1u, v1[u]);
             }
+        }
+        for (; r0 < ch.z; r0 += 256u) {
+            const uint32_t* rp = sc.rec + (size_t)r0 * 3u;
+            add(rp[0], __uint_as_float(rp[1]));
+            add(rp[0] + 1u, __uint_as_float(rp[2]));
+        }
+    } else {
+        for (uint32_t r = ch.y + threadIdx.x; r < ch.z; r += 256u) {
+            const uint32_t* rp = sc.rec + (size_t)r * (1u + F);
+            for (int f = 0; f < F; ++f) add(rp[0] + f, __uint_as_float(rp[1 + f]));
         }
     }
     __syncthreads();
     const size_t gbase = ((size_t)c.offsets[l] + e0) * F;
     if (ch.w == 1u) {
-        for (uint32_t i = threadIdx.x; i < words; i += 256u) {
-            if (DET) grad[gbase + i] = ldexpf((float)(long long)tile[i], -s);
-            else grad[gbase + i] = (float)tile[i];
-        }
+        for (uint32_t i = threadIdx.x; i < words; i += 256u) grad[gbase + i] = ldexpf((float)(long long)tile[i], -s);
     } else {
         for (uint32_t i = threadIdx.x; i < words; i += 256u) {
             if (tile[i] == (Acc)0) continue;
-            if (DET) atomicAdd(&sc.wide[gbase + i], (unsigned long long)tile[i]);
-            else atomicAdd(&grad[gbase + i], (float)tile[i]);
+            if (DET) atomicAdd(&sc.wide[gbase + i], tile[i]);
+            else atomicAdd(&grad[gbase + i], ldexpf((float)(long long)tile[i], -s));
         }
     }
 }
@@ -318,8 +368,7 @@ Scratch carve(const DevCfg& c, const BinLayout& b, uint32_t n_bins, uint32_t n, 
     uint4* ct = cv.take<uint4>((size_t)b.max_chunks + 1u);
     sc.chunks = ct ? ct + 1 : nullptr;
     sc.maxbits = cv.take<uint32_t>(32);
-    sc.rec_idx = cv.take<uint32_t>(recs);
-    sc.rec_val = cv.take<float>(recs * c.F);
+    sc.rec = cv.take<uint32_t>(recs * (1u + c.F));
     *bytes = cv.used;
     return sc;
 }
@@ -355,12 +404,8 @@ hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dg
     for (int l = 0; l < c.n_levels; ++l) bmax = std::max(bmax, b.bin_off[l + 1] - b.bin_off[l]);
     const size_t hist_lds = (size_t)bmax * 4u;
     const dim3 grid(b.nblk, c.n_levels);
-    if (deterministic) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, sc.maxbits, 32u);
-        hipLaunchKernelGGL(k_bin_count<true>, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
-    } else {
-        hipLaunchKernelGGL(k_bin_count<false>, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
-    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, sc.maxbits, 32u);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
     hipLaunchKernelGGL(k_bin_scan, dim3(n_bins), dim3(256), 0, s, b, sc);
     hipLaunchKernelGGL(k_bin_chunks, dim3(1), dim3(256), 0, s, b, n_bins, sc);
     hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
         if (tgt_dir) {
             const float loss = mixture_adjoint<K>(
                 c, [&](int jj) { return out[jj]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.value(),
+                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
                 [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
             ws.loss[q] = loss;
         }
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __
     if (q >= n) return;
     loss[q] = mixture_adjoint<K>(
         c, [&](int j) { return raw[(size_t)j * n + q]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.value(),
+        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
         [&](int j, float v) { dout[(size_t)j * n + q] = v; });
 }
 
@@ -272,11 +272,23 @@ __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restric
         w = w - eff * m1;
         master[i] = w;
     }
+    float e = w;
     if (a.ema) {
         const uint32_t t = a.st->step + 1u, last = a.ema_pow_len - 1u;
         const float deb_old = 1 - a.ema_pow[t - 1u < last ? t - 1u : last];
         const float deb_new = 1.0f / (1 - a.ema_pow[t < last ? t : last]);
-        ema[i] = (ema[i] * a.ema_decay * deb_old + w * (1 - a.ema_decay)) * deb_new;
+        e = (ema[i] * a.ema_decay * deb_old + w * (1 - a.ema_decay)) * deb_new;
+        ema[i] = e;
+    }
+    if (a.half_precision != PREC_FP32 && i >= a.n_matrix) {
+        const size_t j = i - a.n_matrix;
+        if (a.half_precision == PREC_BF16) {
+            static_cast<__bf16*>(a.half_train)[j] = (__bf16)w;
+            static_cast<__bf16*>(a.half_infer)[j] = (__bf16)e;
+        } else {
+            static_cast<_Float16*>(a.half_train)[j] = (_Float16)w;
+            static_cast<_Float16*>(a.half_infer)[j] = (_Float16)e;
+        }
     }
 }
 
@@ -307,17 +319,35 @@ __global__ void __launch_bounds__(256) k_transpose_mlp(const DevCfg c, const flo
     wt[c.layer_off[l] + ii * ow + o] = w[i];
 }
 
-__global__ void __launch_bounds__(1024) k_mean(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
-    __shared__ float red[1024];
+// Sum of x[first, last) in a fixed order: per-thread strided sums, then a tree.
+__device__ __forceinline__ float block_sum(const float* __restrict__ x, uint32_t first, uint32_t last, float* red) {
     float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += 1024) s += x[i];
+    for (uint32_t i = first + threadIdx.x; i < last; i += 256u) s += x[i];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = n ? red[0] / (float)n : 0.0f;
+    return red[0];
+}
+
+// Stage 1: 256 blocks, block b sums its contiguous slice into partials[b].
+__global__ void __launch_bounds__(256) k_mean_partial(const float* __restrict__ x, uint32_t n,
+                                                      float* __restrict__ partials) {
+    __shared__ float red[256];
+    const uint32_t per = (n + 255u) / 256u;
+    const uint32_t lo = blockIdx.x * per < n ? blockIdx.x * per : n, hi = lo + per < n ? lo + per : n;
+    const float s = block_sum(x, lo, hi, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Stage 2: one block, the 256 partials -> the mean.
+__global__ void __launch_bounds__(256) k_mean_final(const float* __restrict__ partials, uint32_t n,
+                                                    float* __restrict__ out) {
+    __shared__ float red[256];
+    const float s = block_sum(partials, 0u, 256u, red);
+    if (threadIdx.x == 0) out[0] = n ? s / (float)n : 0.0f;
 }
 
 template <int W, int K, bool LDS_W>
@@ -441,8 +471,9 @@ hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_mean, dim3(1), dim3(1024), 0, s, x, n, out);
+hipError_t launch_mean(const float* x, uint32_t n, float* partials, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mean_partial, dim3(256), dim3(256), 0, s, x, n, partials);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(256), 0, s, partials, n, out);
     return hipGetLastError();
 }
 

@@ -283,53 +283,97 @@ __global__ void __launch_bounds__(256) k_train_bwd_mfma(const DevCfg c, const T*
 }
 
 // Weight gradients on MFMA: dW_l[o][i] = sum_q dY_l[o][q] X_l[i][q] with the
-// batch as the K dimension. Block = (layer, batch split); its 4 waves share
-// the layer's 32 x 32 output tiles. A and B fragments are 8 consecutive
-// samples of one row, i.e. 16-byte loads from the [dim][n] buffers.
+// batch as the K dimension. Block = (layer, batch split), wave = a quarter of
+// the split's samples; a wave accumulates every 32 x 32 tile of two output
+// tiles at once (up to 2 x 4 tiles, 128 accumulators), so each 16-sample step
+// loads its dY and X fragments once: 8 consecutive samples of one row per
+// lane, i.e. 16-byte loads from the [dim][n] buffers. Each wave writes its own
+// partial row; a deterministic reduction over the rows follows.
 template <class T>
 __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainMfmaWS ws, float* __restrict__ partials,
-                                                    uint32_t n, uint32_t q_per_split, uint32_t n_mlp) {
+                                                    uint32_t n, uint32_t q_per_wave, uint32_t n_mlp) {
     using V = typename mf::Frag<T>::vec;
+    constexpr int TI = 4;  // input tiles per pass
     const int l = blockIdx.x;
     const int H = c.n_layers - 1;
     const int iw = c.layer_in[l], ow = c.layer_out[l];
     const T* X = l == 0 ? static_cast<const T*>(ws.x0) : static_cast<const T*>(ws.xh) + (size_t)(l - 1) * c.width * n;
     const T* G = l == H ? static_cast<const T*>(ws.dyo) : static_cast<const T*>(ws.dyh) + (size_t)l * c.width * n;
-    const uint32_t qb = blockIdx.y * q_per_split;
-    const uint32_t qe = min(n, qb + q_per_split);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
-    const int n_to = ow / 32, n_ti = (iw + 31) / 32, n_pairs = n_to * n_ti;
-    float* P = partials + (size_t)blockIdx.y * n_mlp + c.layer_off[l];
-    for (int pair = wave; pair < n_pairs; pair += 4) {
-        const int to = pair / n_ti, ti = pair % n_ti;
-        // Input rows beyond iw (a partial last tile) read row iw - 1 and are not stored.
-        const int i = 32 * ti + j;
-        const T* ga = G + (size_t)(32 * to + j) * n;
-        const T* xb = X + (size_t)(i < iw ? i : iw - 1) * n;
-        mf::f32x16 acc = {};
-        uint32_t q = qb;
-        // 16-byte fragment loads need 16-byte aligned rows: n a multiple of 8.
-        for (; (n & 7u) == 0u && q + 16u <= qe; q += 16u) {
-            const V a = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(ga + q + 8u * h));
-            const V b = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(xb + q + 8u * h));
-            acc = mf::Frag<T>::mma(a, b, acc);
-        }
-        for (; q < qe; q += 16u) {  // tail (and unaligned batches): element loads with bounds
-            V a, b;
+    const uint32_t row = blockIdx.y * 4u + (uint32_t)wave;
+    const uint32_t qb = row * q_per_wave;
+    const uint32_t qe = min(n, qb + q_per_wave);
+    const int n_to = ow / 32, n_ti = (iw + 31) / 32;
+    float* P = partials + (size_t)row * n_mlp + c.layer_off[l];
+    // 16-byte fragment loads need 16-byte aligned rows: n a multiple of 8.
+    const bool vec = (n & 7u) == 0u;
+    for (int to0 = 0; to0 < n_to; to0 += 2) {
+        const int nto = n_to - to0 < 2 ? n_to - to0 : 2;
+        for (int ti0 = 0; ti0 < n_ti; ti0 += TI) {
+            mf::f32x16 acc[2][TI];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint32_t qq = q + 8u * h + e;
-                a[e] = qq < qe ? ga[qq] : (T)0.0f;
-                b[e] = qq < qe ? xb[qq] : (T)0.0f;
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < TI; ++b) acc[a][b] = mf::f32x16{};
+            const T* ga[2];
+            const T* xb[TI];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) ga[a] = G + (size_t)(32 * (to0 + (a < nto ? a : 0)) + j) * n;
+#pragma unroll
+            for (int b = 0; b < TI; ++b) {
+                // rows beyond iw (a partial or absent tile) read row iw - 1 and are not stored
+                const int i = 32 * (ti0 + b) + j;
+                xb[b] = X + (size_t)(i < iw ? i : iw - 1) * n;
             }
-            acc = mf::Frag<T>::mma(a, b, acc);
-        }
-        if (i < iw) {
+            auto step = [&](uint32_t q, bool full) {
+                V av[2], bv[TI];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int o = 32 * to + 8 * (r >> 2) + 4 * h + (r & 3);
-                P[(size_t)o * iw + i] = acc[r];
+                for (int a = 0; a < 2; ++a) {
+                    if (full) {
+                        av[a] = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(ga[a] + q + 8u * h));
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const uint32_t qq = q + 8u * h + e;
+                            av[a][e] = qq < qe ? ga[a][qq] : (T)0.0f;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < TI; ++b) {
+                    if (full) {
+                        bv[b] = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(xb[b] + q + 8u * h));
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const uint32_t qq = q + 8u * h + e;
+                            bv[b][e] = qq < qe ? xb[b][qq] : (T)0.0f;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < TI; ++b) acc[a][b] = mf::Frag<T>::mma(av[a], bv[b], acc[a][b]);
+            };
+            uint32_t q = qb;
+            if (vec)
+                for (; q + 16u <= qe; q += 16u) step(q, true);
+            for (; q < qe; q += 16u) step(q, false);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                if (a >= nto) break;
+#pragma unroll
+                for (int b = 0; b < TI; ++b) {
+                    const int i = 32 * (ti0 + b) + j;
+                    if (i >= iw) continue;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int o = 32 * (to0 + a) + 8 * (r >> 2) + 4 * h + (r & 3);
+                        P[(size_t)o * iw + i] = acc[a][b][r];
+                    }
+                }
             }
         }
     }
@@ -478,29 +522,39 @@ size_t train_mask_words(const DevCfg& c, uint32_t n) {
     return (size_t)((n + 63u) / 64u) * (size_t)c.n_hidden * (size_t)(c.width / 32) * 64u;
 }
 
+namespace {
+// Batch splitting of the MFMA weight gradient: one partial row per wave,
+// 4 waves per block, at least 1024 samples per wave and at most 1024 rows.
+void wgrad_split(uint32_t n, uint32_t* rows, uint32_t* q_per_wave) {
+    uint32_t r = (n + 1023u) / 1024u;
+    r = (r + 3u) / 4u * 4u;
+    if (r > 1024u) r = 1024u;
+    if (r < 4u) r = 4u;
+    *rows = r;
+    *q_per_wave = ((n + r - 1u) / r + 15u) / 16u * 16u;
+}
+}  // namespace
+
 hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& ws, float* partials,
                              size_t partials_cap, float* dw, uint32_t n, uint32_t* overflow, hipStream_t s) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
-    uint32_t splits = (n + 4095u) / 4096u;
-    if (splits > 256u) splits = 256u;
-    if (splits < 1u) splits = 1u;
-    const uint32_t qps = ((n + splits - 1u) / splits + 15u) / 16u * 16u;
-    if ((size_t)splits * n_mlp > partials_cap) return hipErrorInvalidValue;
-    const dim3 grid(c.n_layers, splits);
-    if (precision == PREC_BF16) hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, partials, n, qps, n_mlp);
-    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws, partials, n, qps, n_mlp);
+    uint32_t rows, qpw;
+    wgrad_split(n, &rows, &qpw);
+    if ((size_t)rows * n_mlp > partials_cap) return hipErrorInvalidValue;
+    const dim3 grid(c.n_layers, rows / 4u);
+    if (precision == PREC_BF16) hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
+    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
     else return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_reduce_splits(partials, splits, n_mlp, dw, s, overflow);
+    return launch_reduce_splits(partials, rows, n_mlp, dw, s, overflow);
 }
 
 size_t wgrad_mfma_partials_needed(const DevCfg& c, uint32_t n) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
-    uint32_t splits = (n + 4095u) / 4096u;
-    if (splits > 256u) splits = 256u;
-    if (splits < 1u) splits = 1u;
-    return (size_t)splits * n_mlp;
+    uint32_t rows, qpw;
+    wgrad_split(n, &rows, &qpw);
+    return (size_t)rows * n_mlp;
 }
 
 size_t bfrag_elements(const DevCfg& c) { return (size_t)n_bfrag_of(c) * 512u; }

@@ -33,12 +33,19 @@ struct TrainWS {
     float* dgrid; // [n_levels F][n] dL/d(grid features)
 };
 
-// Loss scale divided by the batch size, as seen by the adjoint: a device scalar
-// (dynamic loss scaling) times a host factor, or the host factor alone.
+// Loss scale seen by the adjoint: a device scalar (dynamic loss scaling) or a
+// host value, and 1 / batch size. The reference clamps the scaled gradient at
+// loss scale 128 (train.cu:116-122); the threshold here is grad_clamp * scale /
+// clamp_scale, so the clamp acts on the same unscaled values at any scale.
 struct LossScale {
     const float* dev;  // device scalar or null
-    float mul;
-    __device__ float value() const { return (dev ? dev[0] : 1.0f) * mul; }
+    float scale;       // used when dev is null
+    float inv_n;
+    __device__ float value() const { return dev ? dev[0] : scale; }
+    __device__ float over_n() const { return value() * inv_n; }
+    __device__ float clamp(const DevCfg& c) const {
+        return c.grad_clamp > 0.0f ? c.grad_clamp * (value() / c.clamp_scale) : 0.0f;
+    }
 };
 
 // Device-resident optimiser/loss-scale state, so the train step needs no host
@@ -104,6 +111,11 @@ struct AdamArgs {
     const ScaleState* st;
     const float* ema_pow;
     uint32_t ema_pow_len;
+    // operand-type copies of the grid table (parameters n_matrix..n) written in
+    // the same pass: precision PREC_BF16 / PREC_FP16, or none for PREC_FP32
+    int half_precision;
+    void* half_train;  // from the updated parameters
+    void* half_infer;  // from the EMA (the updated parameters without EMA)
 };
 hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, float* m, float* v, uint32_t* steps,
                        float* ema, hipStream_t s);
@@ -112,8 +124,8 @@ hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, floa
 hipError_t launch_scale_update(ScaleState* st, bool dynamic, uint32_t window, float max_scale, hipStream_t s);
 // Transposes the MLP weights of every layer ([out][in] -> [in][out]).
 hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipStream_t s);
-// Mean of n values into out[0] (deterministic tree order).
-hipError_t launch_mean(const float* x, uint32_t n, float* out, hipStream_t s);
+// Mean of n values into out[0] (fixed summation order). partials: 256 floats.
+hipError_t launch_mean(const float* x, uint32_t n, float* partials, float* out, hipStream_t s);
 hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
                           const float* tgt_pdf, LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s);
 

@@ -142,7 +142,10 @@ def benchmark_config(size: str = "full", precision: str = "fp32", n_lobes: int =
         "optimizer": {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
                       "l2_reg": 1e-6},
         "mixture": {"otype": "vMF", "n_lobes": n_lobes, "mapping": "softmax", "numerics": "stable"},
-        "loss": {"scale": 128.0, "grad_clamp": 150.0},
+        # FP32: the reference's static scale; BF16/FP16: dynamic from 2^15 (the
+        # gradient clamp follows the scale, so both train the same function)
+        "loss": ({"scale": 128.0, "grad_clamp": 150.0} if precision == "fp32" else
+                 {"scale": 32768.0, "dynamic": True, "scale_window": 1000, "grad_clamp": 150.0}),
         "precision": precision,
     }
 

@@ -617,7 +617,7 @@ void query_batch(const orc_cfg* c, const float* params, const float* in, const f
 // distribution.h:143-181 (gradients_probability), train.h:212-223.
 template <class T>
 void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3], float wipdf, T ls_over_n,
-                 T* g, T* loss) {
+                 T clamp_hi, T* g, T* loss) {
     const bool ref = c->numerics == ORC_NUM_REFERENCE;
     Mixture<T> m;
     build_mixture(c, x, m);
@@ -694,7 +694,7 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
     // train.cu:106-123: zero non-finite gradients, clamp from above.
     for (int j = 0; j < 4 * K; ++j) {
         if (!M<T>::finite(g[j])) g[j] = 0;
-        if (c->grad_clamp > 0) g[j] = std::min((T)c->grad_clamp, g[j]);
+        if (clamp_hi > 0) g[j] = std::min(clamp_hi, g[j]);
     }
 }
 
@@ -703,11 +703,15 @@ void adjoint_batch(const orc_cfg* c, const T* raw, const float* tgt_dir, const f
                    int n, T loss_scale, T* dout, T* loss) {
     const int n_out = orc_padded_output(c);
     const T ls = loss_scale / (T)n;
+    // The reference clamps the scaled gradient at loss scale 128 (train.cu:116-122,
+    // parameters.h:24); the threshold follows the scale so that the clamp acts on
+    // the same unscaled values at any loss scale.
+    const T clamp_hi = c->grad_clamp > 0 ? (T)c->grad_clamp * (loss_scale / (T)c->clamp_scale) : (T)0;
     std::vector<T> x(n_out), g(n_out);
     for (int q = 0; q < n; ++q) {
         for (int j = 0; j < n_out; ++j) x[j] = raw[(size_t)j * n + q];
         const float rgb[3] = {tgt_rgb[q], tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q]};
-        adjoint_one<T>(c, x.data(), (T)tgt_dir[q], (T)tgt_dir[(size_t)n + q], rgb, tgt_pdf[q], ls, g.data(), &loss[q]);
+        adjoint_one<T>(c, x.data(), (T)tgt_dir[q], (T)tgt_dir[(size_t)n + q], rgb, tgt_pdf[q], ls, clamp_hi, g.data(), &loss[q]);
         for (int j = 0; j < n_out; ++j) dout[(size_t)j * n + q] = g[j];
     }
 }

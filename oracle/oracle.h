@@ -47,6 +47,8 @@ typedef struct {
     int mixture, n_lobes, mapping, numerics;
     float cosine_kappa;
     float grad_clamp;               /* 0 disables the upper clamp */
+    float clamp_scale;              /* loss scale at which grad_clamp applies (the reference's 128):
+                                       at loss scale s the threshold is grad_clamp * s / clamp_scale */
 } orc_cfg;
 
 /* Layout of the network: sizes of the padded encoding, MLP and grid. */

@@ -29,7 +29,7 @@ class OrcCfg(ctypes.Structure):
         ("base_resolution", ctypes.c_int), ("per_level_scale", ctypes.c_float), ("interpolation", ctypes.c_int),
         ("width", ctypes.c_int), ("n_hidden_layers", ctypes.c_int), ("n_output_dims", ctypes.c_int),
         ("mixture", ctypes.c_int), ("n_lobes", ctypes.c_int), ("mapping", ctypes.c_int), ("numerics", ctypes.c_int),
-        ("cosine_kappa", ctypes.c_float), ("grad_clamp", ctypes.c_float),
+        ("cosine_kappa", ctypes.c_float), ("grad_clamp", ctypes.c_float), ("clamp_scale", ctypes.c_float),
     ]
 
 
@@ -78,6 +78,7 @@ def make_cfg(config: dict) -> OrcCfg:
     c.numerics = 0 if mix.get("numerics", "stable").lower() == "reference" else 1
     c.cosine_kappa = float(mix.get("cosine_kappa", 2.18853))
     c.grad_clamp = float(config.get("loss", {}).get("grad_clamp", 150.0))
+    c.clamp_scale = float(config.get("loss", {}).get("clamp_scale", 128.0))
     return c
 
 

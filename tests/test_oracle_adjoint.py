@@ -5,12 +5,12 @@ import pytest
 import pyoracle
 
 
-def _cfg(K=8, mapping="reference", numerics="stable", clamp=0.0):
+def _cfg(K=8, mapping="reference", numerics="stable", clamp=0.0, clamp_scale=128.0):
     return pyoracle.make_cfg({
         "encoding": {"otype": "Composite", "nested": [{"otype": "Identity", "n_dims_to_encode": 9}]},
         "network": {"n_neurons": 64, "n_hidden_layers": 1},
         "mixture": {"n_lobes": K, "mapping": mapping, "numerics": numerics},
-        "loss": {"grad_clamp": clamp}})
+        "loss": {"grad_clamp": clamp, "clamp_scale": clamp_scale}})
 
 
 def _targets(n, seed):
@@ -108,7 +108,22 @@ def test_nonfinite_zeroed_and_upper_clamp(oracle):
     raw = _raw(K, n, 5, "reference")
     td, tr, tp = _targets(n, 6)
     tr[:, :8] = np.inf  # f/q infinite -> non-finite gradients are zeroed (train.cu:106-114)
-    g, _ = oracle.adjoint(_cfg(K, clamp=150.0), raw, td, tr, tp, 1e9, dtype=np.float64)
+    g, _ = oracle.adjoint(_cfg(K, clamp=150.0, clamp_scale=1e9), raw, td, tr, tp, 1e9, dtype=np.float64)
     assert np.all(np.isfinite(g)) and np.all(g[:, :8] == 0)
     assert g.max() <= 150.0 and g.max() == 150.0  # min(150, g) (train.cu:116-122); no lower clamp
     assert g.min() < -150.0
+
+
+def test_clamp_follows_the_loss_scale(oracle):
+    """At loss scale s the clamp threshold is grad_clamp s / clamp_scale: the
+    scaled gradient divided by s is the same at every scale, clamped entries
+    included (the reference fixes s = 128 and clamps at 150)."""
+    K, n = 8, 64
+    raw = _raw(K, n, 5, "reference")
+    td, tr, tp = _targets(n, 6)
+    cfg = _cfg(K, clamp=150.0)
+    g128, _ = oracle.adjoint(cfg, raw, td, tr, tp, 128.0 * 1e6, dtype=np.float64)
+    g32k, _ = oracle.adjoint(cfg, raw, td, tr, tp, 32768.0 * 1e6, dtype=np.float64)
+    hi = 150.0 * 1e6
+    assert np.sum(g128 == hi) > 0, "some entries must hit the clamp"
+    assert np.allclose(g32k / 256.0, g128, rtol=1e-12, atol=1e-200)  # atol: FP64 subnormals
