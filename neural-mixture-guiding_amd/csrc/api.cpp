@@ -26,8 +26,11 @@
 namespace npmg {
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
-hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
-                             const QueryIO& io, uint32_t n, int n_cu, hipStream_t s);
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const void* genc,
+                             const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu,
+                             hipStream_t s);
+hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s);
 bool mfma_split_grid(const DevCfg& c);
 hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
@@ -167,6 +170,10 @@ struct npmg_ctx {
     size_t scatter_cap = 0;
     void* genc = nullptr;  // level-major grid features [L F][n] of the MFMA paths
     uint32_t genc_cap = 0;
+    // second stream and events: the grid encode of chunk c + 1 overlaps the fused kernel of chunk c
+    hipStream_t aux = nullptr;
+    static constexpr int kPipe = 4;
+    hipEvent_t ev_fork = nullptr, ev_chunk[kPipe] = {};
     // MFMA training: transposed weight fragments and the operand-type buffers
     void* bfrag_train = nullptr;
     TrainMfmaWS mws{};
@@ -179,6 +186,10 @@ struct npmg_ctx {
     const float* infer_params() const { return ema ? ema : master; }
 
     ~npmg_ctx() {
+        if (aux) (void)hipStreamDestroy(aux);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (hipEvent_t e : ev_chunk)
+            if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
@@ -229,11 +240,39 @@ struct npmg_ctx {
     // Fused query with the training (0) or inference (1) parameters in the configured precision.
     void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s) {
         if (half()) {
+            const void* frag = inference ? frag_infer : frag_train;
+            const void* tab = inference ? htab_infer : htab_train;
+            if (!mfma_split_grid(cfg.d)) {
+                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, s),
+                          "query (mfma)");
+                return;
+            }
             ensure_genc(n);
-            hip_check(launch_query_mfma(cfg.d, cfg.precision, inference ? frag_infer : frag_train,
-                                        inference ? htab_infer : htab_train, mfma_split_grid(cfg.d) ? genc : nullptr,
-                                        io, n, n_cu, s),
-                      "query (mfma)");
+            // Large batches: kPipe chunks; the encodes run on aux (after everything
+            // already queued on s, which may still read genc), chunk c's fused
+            // kernel on s after chunk c's encode.
+            const uint32_t chunks = n >= (1u << 18) ? (uint32_t)kPipe : 1u;
+            const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
+            if (chunks == 1u) {
+                hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s), "grid encode");
+                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, s),
+                          "query (mfma)");
+                return;
+            }
+            hip_check(hipEventRecord(ev_fork, s), "event");
+            hip_check(hipStreamWaitEvent(aux, ev_fork, 0), "stream wait");
+            for (uint32_t k = 0; k < chunks; ++k) {
+                const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
+                hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux),
+                          "grid encode");
+                hip_check(hipEventRecord(ev_chunk[k], aux), "event");
+            }
+            for (uint32_t k = 0; k < chunks; ++k) {
+                const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
+                hip_check(hipStreamWaitEvent(s, ev_chunk[k], 0), "stream wait");
+                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, s),
+                          "query (mfma)");
+            }
         } else {
             const float* p = inference ? infer_params() : master;
             hip_check(launch_query_fp32(cfg.d, inference ? wt_infer : wt_train, p + cfg.n_mlp, io, n, n_cu, s),
@@ -424,6 +463,9 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        hip_check(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking), "stream");
+        hip_check(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
+        for (hipEvent_t& e : ctx->ev_chunk) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         const Config& c = ctx->cfg;
         ctx->master = dalloc<float>(c.n_params);
         if (c.ema) ctx->ema = dalloc<float>(c.n_params);

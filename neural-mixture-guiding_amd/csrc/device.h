@@ -181,14 +181,37 @@ __device__ __forceinline__ int level_corners(const DevCfg& c, int l, float x0, f
 // tiny-cuda-nn common_device.h:533-548. The kernel's CDF is monotonic and
 // reaches its clamps; beyond |u| >= 1.0625 the clamped result is exactly 0 or
 // 1, so those evaluations are skipped (bit-identical).
-__device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
-    const float u = x * inv_radius;
-    if (u <= -1.0625f) return 0.0f;
-    if (u >= 1.0625f) return 1.0f;
+__device__ __forceinline__ float quartic_poly(float u) {
     const float u2 = u * u;
     const float u4 = u2 * u2;
     const float v = 15.0f / 16.0f * u * (1.0f - 2.0f / 3.0f * u2 + 1.0f / 5.0f * u4) + 0.5f;
     return smax(0.0f, smin(1.0f, v));
+}
+__device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
+    const float u = x * inv_radius;
+    if (u <= -1.0625f) return 0.0f;
+    if (u >= 1.0625f) return 1.0f;
+    return quartic_poly(u);
+}
+
+// One OneBlob bin edge (oneblob.h:114-123): the sum of the three wrapped
+// quartic CDFs cdf(d) + cdf(d - 1) + cdf(d + 1), d = edge - x, in that order.
+// The three arguments are n_bins apart in u, so for n_bins >= 4 at most one
+// of them lies inside the support |u| < 1.0625: the polynomial is evaluated
+// once, at that argument, and the other two take their exact clamp values
+// (bit-identical to three quartic_cdf calls).
+__device__ __forceinline__ float oneblob_edge(float d, float fnb) {
+    if (fnb < 4.0f)
+        return quartic_cdf(d, fnb) + quartic_cdf(d - 1.0f, fnb) + quartic_cdf(d + 1.0f, fnb);
+    const float u1 = d * fnb, u2 = (d - 1.0f) * fnb, u3 = (d + 1.0f) * fnb;
+    const bool in1 = u1 > -1.0625f && u1 < 1.0625f;
+    const bool in2 = u2 > -1.0625f && u2 < 1.0625f;
+    const bool in3 = u3 > -1.0625f && u3 < 1.0625f;
+    const float p = quartic_poly(in1 ? u1 : (in2 ? u2 : u3));
+    const float c1 = in1 ? p : (u1 <= -1.0625f ? 0.0f : 1.0f);
+    const float c2 = in2 ? p : (u2 <= -1.0625f ? 0.0f : 1.0f);
+    const float c3 = in3 ? p : (u3 <= -1.0625f ? 0.0f : 1.0f);
+    return c1 + c2 + c3;
 }
 
 // q_in[d] for a runtime d without indexing private memory (keeps q_in in
@@ -254,13 +277,12 @@ __device__ __forceinline__ void encode_query(const DevCfg& c, const T* __restric
         } else if (type == ENC_ONEBLOB) {
             const int nb = c.enc_param[e];
             const float fnb = (float)nb;
+            const float inv_nb = 1.0f / fnb;  // n_bins is a power of two: k * inv_nb == k / n_bins exactly
             for (int d = 0; d < c.enc_ndims[e]; ++d) {
                 const float xv = pick(q_in, b + d);
-                float left = quartic_cdf(-xv, fnb) + quartic_cdf(-xv - 1.0f, fnb) + quartic_cdf(-xv + 1.0f, fnb);
+                float left = oneblob_edge(-xv, fnb);
                 for (int k = 0; k < nb; ++k) {
-                    const float rb = (float)(k + 1) / fnb;
-                    const float right =
-                        quartic_cdf(rb - xv, fnb) + quartic_cdf(rb - xv - 1.0f, fnb) + quartic_cdf(rb - xv + 1.0f, fnb);
+                    const float right = oneblob_edge((float)(k + 1) * inv_nb - xv, fnb);
                     feed(right - left);
                     left = right;
                 }
@@ -560,9 +582,7 @@ namespace dev {
 // MFMA kernels see exactly the FP32 path's features before rounding.
 
 // Sum of the three wrapped quartic CDFs at bin boundary b (oneblob.h:114-123).
-__device__ __forceinline__ float oneblob_cdf(float b, float x, float fnb) {
-    return quartic_cdf(b - x, fnb) + quartic_cdf(b - x - 1.0f, fnb) + quartic_cdf(b - x + 1.0f, fnb);
-}
+__device__ __forceinline__ float oneblob_cdf(float b, float x, float fnb) { return oneblob_edge(b - x, fnb); }
 
 // Features of one grid level (F values) accumulated over the corners in order.
 template <class T, int F>
@@ -618,8 +638,8 @@ __device__ __forceinline__ float single_feature(const DevCfg& c, const T* __rest
         const float fnb = (float)nb;
         const int d = r / nb, k = r % nb;
         const float xv = pick(q_in, b + d);
-        const float left = oneblob_cdf((float)k / fnb, xv, fnb);
-        const float right = oneblob_cdf((float)(k + 1) / fnb, xv, fnb);
+        const float left = oneblob_cdf((float)k * (1.0f / fnb), xv, fnb);
+        const float right = oneblob_cdf((float)(k + 1) * (1.0f / fnb), xv, fnb);
         return right - left;
     }
     if (type == ENC_SH) {
@@ -681,10 +701,11 @@ __device__ __forceinline__ void encode_chunk(const DevCfg& c, const T* __restric
         const float fnb = (float)nb;
         const int d = r0 / nb, k0 = r0 % nb;
         const float xv = pick(q_in, b + d);
-        float left = oneblob_cdf((float)k0 / fnb, xv, fnb);
+        const float inv_nb = 1.0f / fnb;  // power of two: exact
+        float left = oneblob_cdf((float)k0 * inv_nb, xv, fnb);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const float right = oneblob_cdf((float)(k0 + i + 1) / fnb, xv, fnb);
+            const float right = oneblob_cdf((float)(k0 + i + 1) * inv_nb, xv, fnb);
             v[i] = right - left;
             left = right;
         }

@@ -2,10 +2,14 @@
 // operands, FP32 accumulation), plus the packing kernels that produce its
 // weight fragments and half-precision grid tables.
 //
+// The grid features come from k_grid_encode (level-major, operand type, [L F][n])
+// when the grid rows form whole 8-feature chunks; the host pipelines it with the
+// fused kernel over chunks of the batch on two streams, since the encode is
+// bound by L2 gather bandwidth and the fused kernel by VALU issue.
 // One wave handles 64 queries. Lane l = 32 h + j encodes, for queries j and
 // 32 + j of the tile, the feature chunks 2 kb + h (8 features each) of every
 // K-block kb of the first layer, and hands them to v_mfma_f32_32x32x16 as B
-// fragments; the encoding never leaves registers. Weights live in LDS in
+// fragments straight from registers. Weights live in LDS in
 // fragment order (1 KB per 32x16 fragment, conflict-free reads). Hidden
 // layers chain register to register (see mfma.h). The output tile is split
 // between lanes l and l ^ 32 and exchanged with one shuffle, after which
@@ -26,10 +30,11 @@ using namespace dev;
 // L2 while it is gathered; the features go out in the operand type, [L F][n].
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __restrict__ table,
-                                                     const float* __restrict__ in, T* __restrict__ genc, uint32_t n) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+                                                     const float* __restrict__ in, T* __restrict__ genc, uint32_t n,
+                                                     uint32_t q_begin, uint32_t q_end) {
+    const uint32_t q = q_begin + blockIdx.x * 256u + threadIdx.x;
     const int l = blockIdx.y;
-    if (q >= n) return;
+    if (q >= q_end) return;
     const int b = c.enc_begin[c.grid_enc];
     const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
     if (c.F == 2) {
@@ -57,10 +62,12 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
     }
 }
 
+// Queries [q_begin, q_end) of SoA arrays with row stride n.
 template <class T, int W, int K>
 __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __restrict__ wfrag,
                                                     const T* __restrict__ table, const T* __restrict__ genc,
-                                                    const QueryIO io, uint32_t n, uint32_t n_frag) {
+                                                    const QueryIO io, uint32_t n, uint32_t n_frag, uint32_t q_begin,
+                                                    uint32_t q_end) {
     using V = typename mf::Frag<T>::vec;
     constexpr int NOUT = (4 * K + 15) / 16 * 16;
     constexpr int TO = NOUT / 32;  // output tiles
@@ -73,10 +80,10 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     const int KB0 = c.enc_padded / 16;
-    const uint32_t n_tiles = (n + 63u) / 64u;
+    const uint32_t n_tiles = (q_end - q_begin + 63u) / 64u;
     for (uint32_t tile = blockIdx.x * 4u + (uint32_t)wave; tile < n_tiles; tile += gridDim.x * 4u) {
-        const uint32_t q0 = tile * 64u;
-        const uint32_t qa = min(q0 + (uint32_t)j, n - 1u), qb = min(q0 + 32u + (uint32_t)j, n - 1u);
+        const uint32_t q0 = q_begin + tile * 64u;
+        const uint32_t qa = min(q0 + (uint32_t)j, q_end - 1u), qb = min(q0 + 32u + (uint32_t)j, q_end - 1u);
         float ina[16], inb[16];
 #pragma unroll
         for (int d = 0; d < 16; ++d) {
@@ -162,7 +169,7 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __r
             }
         }
         const uint32_t q = q0 + (uint32_t)lane;
-        if (q < n) {
+        if (q < q_end) {
             if (io.raw_out) {
 #pragma unroll
                 for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
@@ -226,14 +233,8 @@ __global__ void __launch_bounds__(256) k_to_half(const float* __restrict__ src, 
 }
 
 template <class T, int W, int K>
-hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, void* genc, const QueryIO& io,
-                         uint32_t n, int n_cu, hipStream_t s) {
-    if (genc) {
-        hipLaunchKernelGGL((k_grid_encode<T>), dim3((n + 255) / 256, c.n_levels), dim3(256), 0, s, c,
-                           static_cast<const T*>(table), io.in, static_cast<T*>(genc), n);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const QueryIO& io,
+                         uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, hipStream_t s) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const uint32_t n_frag = n_mlp / 512u;
     const size_t lds = (size_t)n_frag * 1024u;
@@ -244,51 +245,60 @@ hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, 
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
-    const uint32_t tiles = (n + 63u) / 64u;
+    const uint32_t tiles = (q_end - q_begin + 63u) / 64u;
     const uint32_t want = (tiles + 3u) / 4u;
     const uint32_t grid = want < (uint32_t)(n_cu * per_cu) ? want : (uint32_t)(n_cu * per_cu);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, c, static_cast<const T*>(wfrag),
-                       static_cast<const T*>(table), static_cast<const T*>(genc), io, n, n_frag);
+                       static_cast<const T*>(table), static_cast<const T*>(genc), io, n, n_frag, q_begin, q_end);
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, void* genc, const QueryIO& io,
-                            uint32_t n, int n_cu, hipStream_t s) {
-    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, genc, io, n, n_cu, s);
-    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, genc, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, genc, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, genc, io, n, n_cu, s);
+hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, const void* genc,
+                            const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, hipStream_t s) {
+    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
     return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_grid_encode(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
-                              uint32_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const dim3 grid((n + 255) / 256, c.n_levels);
+hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s) {
+    if (q_end <= q_begin) return hipSuccess;
+    const dim3 grid((q_end - q_begin + 255) / 256, c.n_levels);
     if (precision == PREC_BF16)
-        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, in, (__bf16*)genc, n);
+        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, in, (__bf16*)genc,
+                           n, q_begin, q_end);
     else if (precision == PREC_FP16)
         hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, in,
-                           (_Float16*)genc, n);
+                           (_Float16*)genc, n, q_begin, q_end);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+hipError_t launch_grid_encode(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
+                              uint32_t n, hipStream_t s) {
+    return launch_grid_encode_range(c, precision, table, in, genc, n, 0u, n, s);
 }
 
 bool mfma_split_grid(const DevCfg& c) {
     return c.grid_enc >= 0 && c.enc_offset[c.grid_enc] % 8 == 0 && (c.n_levels * c.F) % 8 == 0;
 }
 
-hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
-                             const QueryIO& io, uint32_t n, int n_cu, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    if (c.enc_padded % 16 != 0) return hipErrorInvalidValue;
+// The fused kernel over queries [q_begin, q_end); with genc, the grid features
+// of those queries must already be there (launch_grid_encode_range).
+hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const void* genc,
+                             const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu,
+                             hipStream_t s) {
+    if (q_end <= q_begin) return hipSuccess;
+    if (q_end > n || c.enc_padded % 16 != 0) return hipErrorInvalidValue;
     if (genc && !mfma_split_grid(c)) return hipErrorInvalidValue;
-    if (precision == PREC_BF16) return launch_mfma_type<__bf16>(c, wfrag, table, genc, io, n, n_cu, s);
-    if (precision == PREC_FP16) return launch_mfma_type<_Float16>(c, wfrag, table, genc, io, n, n_cu, s);
+    if (precision == PREC_BF16) return launch_mfma_type<__bf16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+    if (precision == PREC_FP16) return launch_mfma_type<_Float16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
     return hipErrorInvalidValue;
 }
 
