@@ -159,6 +159,26 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
  * 8 sqrt, 9 logistic, 10 x / y. For bitwise host/device comparisons. */
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream);
 
+/* ---- FP64 validation kernels ---------------------------------------------
+ * No counterpart in the reference, which has no tests: these are the checks
+ * its pdf (distribution.h:100-119) and loss adjoint (train.cu:69-124) are
+ * held to, run on the GPU in FP64. Synchronous with respect to the host
+ * (the quadrature rule is computed and uploaded on the first call). */
+
+/* Sphere integral of each query's mixture pdf, as the library evaluates it in
+ * FP32 from raw outputs raw [n_out][n] (with the cosine product about normal
+ * [3][n] when normal is not NULL), accumulated in FP64 over an n_z-point
+ * Gauss-Legendre rule in cos(theta) times n_phi uniform points in phi.
+ * out: [n] device doubles, 1 for a normalised pdf. */
+npmg_status npmg_validate_pdf_integral(npmg_ctx* ctx, const float* raw, const float* normal, uint32_t n, uint32_t n_z,
+                                       uint32_t n_phi, double* out, npmg_stream stream);
+/* Central finite differences with step h, in FP64, of the per-sample loss
+ * -(f/q) log(p + 1e-5) with respect to each raw output (f, q, direction as in
+ * npmg_mixture_adjoint). out: [n_out][n] device doubles; rows past 4 K are 0.
+ * npmg_mixture_adjoint with loss_scale = n returns the same derivatives. */
+npmg_status npmg_validate_loss_fd(npmg_ctx* ctx, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                                  const float* tgt_pdf, uint32_t n, double h, double* out, npmg_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

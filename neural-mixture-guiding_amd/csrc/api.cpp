@@ -45,6 +45,11 @@ hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& 
 size_t wgrad_mfma_partials_needed(const DevCfg& c, uint32_t n);
 size_t train_mask_words(const DevCfg& c, uint32_t n);
 size_t bfrag_elements(const DevCfg& c);
+hipError_t launch_pdf_integral(const DevCfg& c, const float* raw, const float* normal, const double* gz,
+                               const double* gw, uint32_t n_z, uint32_t n_phi, double* out, uint32_t n,
+                               hipStream_t s);
+hipError_t launch_loss_fd(const DevCfg& c, const float* raw, const float* td, const float* tr, const float* tp,
+                          double h, double* out, uint32_t n, hipStream_t s);
 hipError_t launch_pack_bfrags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 }  // namespace npmg
 
@@ -180,6 +185,8 @@ struct npmg_ctx {
     // loss scale, overflow flag and step count on the device; decay^t table of the EMA debiasing
     ScaleState* sstate = nullptr;
     float* mean_partials = nullptr;  // [256]
+    double* gl_nodes = nullptr;      // Gauss-Legendre rule of the quadrature check: nodes, then weights
+    uint32_t gl_n = 0;
     float* ema_pow = nullptr;
     uint32_t ema_pow_len = 0;
 
@@ -194,7 +201,7 @@ struct npmg_ctx {
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
-                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials})
+                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes})
             if (p) (void)hipFree(p);
     }
 
@@ -438,6 +445,41 @@ npmg_status guarded(npmg_ctx* ctx, F&& f) {
 }
 
 hipStream_t S(npmg_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Gauss-Legendre rule on [-1, 1]: n nodes then n weights (Newton iteration on
+// P_n from the Chebyshev initial guesses, the textbook construction).
+std::vector<double> gauss_legendre(uint32_t n) {
+    std::vector<double> r(2 * (size_t)n);
+    const double pi = 3.14159265358979323846;
+    for (uint32_t i = 0; i < (n + 1) / 2; ++i) {
+        double x = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (uint32_t k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1.0);
+            const double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = x;
+        for (uint32_t k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1.0);
+        const double w = 2.0 / ((1.0 - x * x) * dp * dp);
+        r[i] = -x;
+        r[n - 1 - i] = x;
+        r[n + i] = w;
+        r[n + n - 1 - i] = w;
+    }
+    return r;
+}
 
 void need(bool ok, const char* msg) {
     if (!ok) throw std::invalid_argument(msg);
@@ -720,6 +762,41 @@ npmg_status npmg_train_state(const npmg_ctx* ctx, float* loss_scale, uint32_t* s
         hip_check(hipMemcpy(&st, ctx->sstate, sizeof(st), hipMemcpyDeviceToHost), "copy state");
         if (loss_scale) *loss_scale = st.scale;
         if (step) *step = st.step;
+    });
+}
+
+npmg_status npmg_validate_pdf_integral(npmg_ctx* ctx, const float* raw, const float* normal, uint32_t n, uint32_t n_z,
+                                       uint32_t n_phi, double* out, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n == 0 || (raw && out), "null buffer");
+        need(n_z >= 2 && n_z <= 1u << 16 && n_phi >= 1 && n_phi <= 1u << 20, "quadrature size out of range");
+        need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the validation kernels support 8 or 16 lobes");
+        if (ctx->gl_n != n_z) {
+            std::vector<double> r = gauss_legendre(n_z);
+            if (ctx->gl_nodes) hip_check(hipFree(ctx->gl_nodes), "hipFree");
+            ctx->gl_nodes = nullptr;
+            ctx->gl_n = 0;
+            ctx->gl_nodes = dalloc<double>(r.size());
+            hip_check(hipMemcpy(ctx->gl_nodes, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+            ctx->gl_n = n_z;
+        }
+        hip_check(launch_pdf_integral(ctx->cfg.d, raw, normal, ctx->gl_nodes, ctx->gl_nodes + n_z, n_z, n_phi, out, n,
+                                      S(stream)),
+                  "pdf integral");
+        hip_check(hipStreamSynchronize(S(stream)), "synchronize");
+    });
+}
+
+npmg_status npmg_validate_loss_fd(npmg_ctx* ctx, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                                  const float* tgt_pdf, uint32_t n, double h, double* out, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && out), "null buffer");
+        need(h > 0, "step must be positive");
+        need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the validation kernels support 8 or 16 lobes");
+        hip_check(launch_loss_fd(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf, h, out, n, S(stream)), "loss fd");
+        hip_check(hipStreamSynchronize(S(stream)), "synchronize");
     });
 }
 

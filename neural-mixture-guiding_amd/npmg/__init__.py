@@ -56,6 +56,8 @@ SIGNATURES = {
     "npmg_synth_queries": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "npmg_synth_train": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "npmg_debug_math": (_int, [_int, _vp, _vp, _u32, _vp, _vp]),
+    "npmg_validate_pdf_integral": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp]),
+    "npmg_validate_loss_fd": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp, _vp]),
 }
 
 
@@ -330,6 +332,27 @@ class GuidingNetwork:
         self._check(self._lib.npmg_train_step(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb), _ptr(tgt_pdf),
                                                n, _ptr(loss_out), _stream()))
         return loss_out
+
+    # -- FP64 validation ---------------------------------------------------------
+    def pdf_integral(self, raw_out, normal=None, n_z: int = 1024, n_phi: int = 2048):
+        """Sphere integral (FP64 quadrature) of each query's mixture pdf."""
+        import torch
+
+        n = raw_out.shape[1]
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        self._check(self._lib.npmg_validate_pdf_integral(self._h, _ptr(raw_out), _ptr(normal), n, n_z, n_phi,
+                                                         _ptr(out), _stream()))
+        return out
+
+    def loss_fd(self, raw_out, tgt_dir, tgt_rgb, tgt_pdf, h: float = 1e-4):
+        """FP64 central differences of the per-sample loss with respect to the raw outputs."""
+        import torch
+
+        n = raw_out.shape[1]
+        out = torch.empty(self.padded_output, n, dtype=torch.float64, device="cuda")
+        self._check(self._lib.npmg_validate_loss_fd(self._h, _ptr(raw_out), _ptr(tgt_dir), _ptr(tgt_rgb),
+                                                    _ptr(tgt_pdf), n, float(h), _ptr(out), _stream()))
+        return out
 
     def train_state(self):
         s, t = _f32(), _u32()
