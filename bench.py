@@ -133,24 +133,45 @@ def main():
     t = npmg.synth_train(77, N_TRAIN)
     loss = torch.empty(1, device="cuda")
 
-    def measure_train(netp, precision):
+    def measure_train(netp, precision, graph=False):
+        fn = lib.npmg_train_step_graph if graph else lib.npmg_train_step
+
         def train_step():
-            st = lib.npmg_train_step(netp._h, P(t["inputs"]), P(t["tgt_dir"]), P(t["tgt_rgb"]), P(t["tgt_pdf"]),
-                                     N_TRAIN, P(loss), stream.cuda_stream)
+            st = fn(netp._h, P(t["inputs"]), P(t["tgt_dir"]), P(t["tgt_rgb"]), P(t["tgt_pdf"]), N_TRAIN, P(loss),
+                    stream.cuda_stream)
             if st != 0:
                 raise npmg.NpmgError(st, lib.npmg_last_error(netp._h).decode())
 
         ms = time_region(train_step, max(2, args.steps // 2), 2, stream)
-        log(f"{precision} train step: {ms:.3f} ms per 2^18 samples ({N_TRAIN / ms / 1e3:.2f} M samples/s), "
-            f"loss {loss.item():.4f}")
+        log(f"{precision} train step{' (graph)' if graph else ''}: {ms:.3f} ms per 2^18 samples "
+            f"({N_TRAIN / ms / 1e3:.2f} M samples/s), loss {loss.item():.4f}")
         return ms
 
-    per_prec, per_prec_train = {}, {}
+    def loss_after(precision, steps=500):
+        """Train a fresh network for `steps` steps on fresh batches of one analytic
+        field (same seeds for every precision), then the mean loss of unseen
+        records of that field with the training weights."""
+        netp = npmg.GuidingNetwork(npmg.benchmark_config("full", precision=precision), seed=7272)
+        batch = npmg.synth_train(123, N_TRAIN)
+        for k in range(steps):
+            lib.npmg_synth_train(123, k * N_TRAIN, N_TRAIN, P(batch["inputs"]), P(batch["tgt_dir"]),
+                                 P(batch["tgt_rgb"]), P(batch["tgt_pdf"]), stream.cuda_stream)
+            netp.train_step_graph(batch["inputs"], batch["tgt_dir"], batch["tgt_rgb"], batch["tgt_pdf"], loss)
+        held = npmg.synth_train(123, N_TRAIN, first=(steps + 1) * N_TRAIN)  # same field, unseen records
+        raw = netp.forward(held["inputs"])
+        _, per = netp.mixture_adjoint(raw, held["tgt_dir"], held["tgt_rgb"], held["tgt_pdf"], 1.0)
+        value = per.mean().item()
+        netp.close()
+        log(f"{precision}: held-out loss after {steps} steps {value:.4f}")
+        return value
+
+    per_prec, per_prec_train, per_prec_graph = {}, {}, {}
     for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
         netp, ms = measure_queries(prec)
         per_prec[prec] = ms
         if args.only != "query":
             per_prec_train[prec] = measure_train(netp, prec)
+            per_prec_graph[prec] = measure_train(netp, prec, graph=True)
         if prec != args.precision:
             netp.close()
         else:
@@ -197,6 +218,8 @@ def main():
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
         "queries_per_s_by_precision": {k: N_QUERIES / (v * 1e-3) for k, v in per_prec.items()},
         "train_samples_per_s_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_train.items()},
+        "train_samples_per_s_graph_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_graph.items()},
+        "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16")},
     }
     if "fp32" in per_prec and half:
         result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]

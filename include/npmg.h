@@ -111,6 +111,14 @@ npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float
  * No host synchronisation. */
 npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
                             const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream);
+/* npmg_train_step as one HIP graph: captured on the first call for these
+ * buffers and this n (the workspace is sized before the capture), replayed by
+ * every later call with the same arguments; other arguments capture anew.
+ * Same results as npmg_train_step (the loss scale and step count live on the
+ * device). Replaces the per-kernel launches of integrator.cpp:650-702 with one
+ * graph launch; the buffers must stay allocated while the graph may replay. */
+npmg_status npmg_train_step_graph(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                                  const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream);
 
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 

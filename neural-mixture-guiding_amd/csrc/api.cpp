@@ -177,6 +177,11 @@ struct npmg_ctx {
     uint32_t genc_cap = 0;
     // second stream and events: the grid encode of chunk c + 1 overlaps the fused kernel of chunk c
     hipStream_t aux = nullptr;
+    // the captured train step (npmg_train_step_graph) and what it was captured for
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* graph_key[6] = {};
     static constexpr int kPipe = 4;
     hipEvent_t ev_fork = nullptr, ev_chunk[kPipe] = {};
     // MFMA training: transposed weight fragments and the operand-type buffers
@@ -193,6 +198,8 @@ struct npmg_ctx {
     const float* infer_params() const { return ema ? ema : master; }
 
     ~npmg_ctx() {
+        drop_graph();
+        if (capture) (void)hipStreamDestroy(capture);
         if (aux) (void)hipStreamDestroy(aux);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         for (hipEvent_t e : ev_chunk)
@@ -220,6 +227,7 @@ struct npmg_ctx {
 
     void ensure_genc(uint32_t n) {
         if (!mfma_split_grid(cfg.d) || n <= genc_cap) return;
+        drop_graph();
         if (genc) hip_check(hipFree(genc), "hipFree");
         genc = nullptr;
         genc_cap = 0;
@@ -288,6 +296,7 @@ struct npmg_ctx {
     }
 
     void free_ws() {
+        drop_graph();  // the captured step points into the workspace
         for (void* p : {(void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss, (void*)ws.dgrid,
                         mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask, (void*)mws.raw})
             if (p) hip_check(hipFree(p), "hipFree");
@@ -389,6 +398,25 @@ struct npmg_ctx {
                       "weight gradient (mfma)");
         else
             hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, of, s), "weight gradient");
+    }
+
+    // One optimisation step on a batch: forward + adjoint, backward, Adam/EMA,
+    // loss mean. Enqueues kernels only (no allocation once ensure_ws(n) ran, no
+    // host synchronisation, no memset nodes), so it can be captured.
+    void train_step(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, float* loss_out,
+                    hipStream_t s) {
+        train_forward(in, td, tr, tp, n, state_scale(n), s);
+        train_backward(in, ws.dout, grad, nullptr, n, cfg.dynamic_scale, s);
+        optimizer_step(grad, 0.0f, s);
+        if (loss_out) hip_check(launch_mean(ws.loss, n, mean_partials, loss_out, s), "loss mean");
+    }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        std::fill(graph_key, graph_key + 6, nullptr);
     }
 
     // Adam + EMA. ls > 0: the gradient is divided by ls (split-pass API);
@@ -506,6 +534,7 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
         hip_check(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking), "stream");
+        hip_check(hipStreamCreateWithFlags(&ctx->capture, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
         for (hipEvent_t& e : ctx->ev_chunk) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         const Config& c = ctx->cfg;
@@ -746,11 +775,35 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
     return guarded(ctx, [&] {
         ctx->check_train();
         need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
-        hipStream_t s = S(stream);
-        ctx->train_forward(in, tgt_dir, tgt_rgb, tgt_pdf, n, ctx->state_scale(n), s);
-        ctx->train_backward(in, ctx->ws.dout, ctx->grad, nullptr, n, ctx->cfg.dynamic_scale, s);
-        ctx->optimizer_step(ctx->grad, 0.0f, s);
-        if (loss_out) hip_check(launch_mean(ctx->ws.loss, n, ctx->mean_partials, loss_out, s), "loss mean");
+        ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, S(stream));
+    });
+}
+
+npmg_status npmg_train_step_graph(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                                  const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_train();
+        need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
+        const void* key[6] = {in, tgt_dir, tgt_rgb, tgt_pdf, loss_out, (const void*)(uintptr_t)n};
+        if (!ctx->graph_exec || !std::equal(key, key + 6, ctx->graph_key)) {
+            ctx->drop_graph();
+            ctx->ensure_ws(n);  // every allocation happens before the capture
+            hip_check(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeRelaxed), "begin capture");
+            hipGraph_t g = nullptr;
+            try {
+                ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, ctx->capture);
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx->capture, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(ctx->capture, &g), "end capture");
+            ctx->graph = g;
+            hip_check(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
+            std::copy(key, key + 6, ctx->graph_key);
+        }
+        hip_check(hipGraphLaunch(ctx->graph_exec, S(stream)), "launch graph");
     });
 }
 

@@ -56,6 +56,7 @@ SIGNATURES = {
     "npmg_synth_queries": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "npmg_synth_train": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "npmg_debug_math": (_int, [_int, _vp, _vp, _u32, _vp, _vp]),
+    "npmg_train_step_graph": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
     "npmg_validate_pdf_integral": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp]),
     "npmg_validate_loss_fd": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp, _vp]),
 }
@@ -353,6 +354,13 @@ class GuidingNetwork:
         self._check(self._lib.npmg_validate_loss_fd(self._h, _ptr(raw_out), _ptr(tgt_dir), _ptr(tgt_rgb),
                                                     _ptr(tgt_pdf), n, float(h), _ptr(out), _stream()))
         return out
+
+    def train_step_graph(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, loss_out):
+        """train_step as one captured HIP graph (replayed while the buffers are the same)."""
+        n = inputs.shape[1]
+        self._check(self._lib.npmg_train_step_graph(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb),
+                                                     _ptr(tgt_pdf), n, _ptr(loss_out), _stream()))
+        return loss_out
 
     def train_state(self):
         s, t = _f32(), _u32()
