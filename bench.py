@@ -28,7 +28,21 @@ N_TRAIN = 1 << 18
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-BYTES_PER_QUERY = 36 + 12 + 48 + 12 + 4 + 16 + 4  # in, u, 4 eval dirs; dir, pdf, 4 pdfs, lobe
+# in, u, normal, 4 eval dirs; dir, pdf, 4 pdfs, lobe
+BYTES_PER_QUERY = 36 + 12 + 12 + 48 + 12 + 4 + 16 + 4
+GRID_FEATURES = 16 * 2  # levels x features of the full configuration
+
+
+def profiled_traffic(precision):
+    """HBM bytes per launch of the fused query kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_query_{precision}_full_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def log(msg):
@@ -128,6 +142,18 @@ def main():
 
         ms = time_region(query_step, args.steps, args.warmup, stream)
         log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
+        # per-kernel launch durations: HIP events recorded by the library around each
+        # launch, on the stream that launch runs on (a separate pass, not the timed one)
+        net.timing(True)
+        for _ in range(args.steps):
+            query_step()
+        kern = {}
+        for tag, name in ((npmg.NPMG_TIME_QUERY, "query"), (npmg.NPMG_TIME_GRID_ENCODE, "grid_encode")):
+            tot, cnt = net.timing_read(tag)
+            if cnt:
+                kern[name] = {"avg_ms_per_launch": tot / cnt, "launches_per_step": cnt / args.steps}
+        net.timing(False)
+        per_kernel[precision] = kern
         return net, ms
 
     t = npmg.synth_train(77, N_TRAIN)
@@ -165,7 +191,7 @@ def main():
         log(f"{precision}: held-out loss after {steps} steps {value:.4f}")
         return value
 
-    per_prec, per_prec_train, per_prec_graph = {}, {}, {}
+    per_prec, per_prec_train, per_prec_graph, per_kernel = {}, {}, {}, {}
     for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
         netp, ms = measure_queries(prec)
         per_prec[prec] = ms
@@ -186,11 +212,17 @@ def main():
     ms_t = per_prec_train[args.precision]
     sps = N_TRAIN / (ms_t * 1e-3)
 
-    table_bytes = net.n_grid * (4 if args.precision == "fp32" else 2)
     half = args.precision != "fp32"
-    alg_bytes = BYTES_PER_QUERY * N_QUERIES + table_bytes
-    achieved = alg_bytes / (ms_q * 1e-3) / 1e9
-    flops = 2 * net.n_mlp * N_QUERIES
+    # Roofline of the dominant kernel, the fused query kernel: algorithmic bytes of
+    # one launch (its share of the queries) over its average launch duration.
+    kq = per_kernel[args.precision]["query"]
+    q_per_launch = N_QUERIES / kq["launches_per_step"]
+    genc_bytes = (GRID_FEATURES * 2) if half else 0  # level-major grid features read back (BF16)
+    table_bytes = 0 if half else net.n_grid * 4       # FP32 path gathers the table itself
+    alg_bytes = (BYTES_PER_QUERY + genc_bytes) * q_per_launch + table_bytes
+    achieved = alg_bytes / (kq["avg_ms_per_launch"] * 1e-3) / 1e9
+    flops = 2 * net.n_mlp * q_per_launch
+    traffic = profiled_traffic(args.precision)
     result = {
         "metric": METRIC,
         "value": qps,
@@ -210,11 +242,14 @@ def main():
         "train_samples_per_s": sps,
         "train_ms_per_step": ms_t,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_query_mfma" if half else "k_query_fp32") +
-                               " (fused encode+MLP+mixture+sample+pdf)",
-                     "algorithmic_bytes_per_launch": alg_bytes},
-        "mlp_compute": {"flops_per_launch": flops, "achieved_tflops": flops / (ms_q * 1e-3) / 1e12,
+                               " (fused OneBlob encode + MLP + mixture + sample + pdf)",
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": kq["avg_ms_per_launch"], "launches_per_step": kq["launches_per_step"]},
+        "kernel_launches": per_kernel,
+        "mlp_compute": {"flops_per_launch": flops,
+                        "achieved_tflops": flops / (kq["avg_ms_per_launch"] * 1e-3) / 1e12,
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
         "queries_per_s_by_precision": {k: N_QUERIES / (v * 1e-3) for k, v in per_prec.items()},
         "train_samples_per_s_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_train.items()},

@@ -167,6 +167,21 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
  * 8 sqrt, 9 logistic, 10 x / y. For bitwise host/device comparisons. */
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream);
 
+/* ---- Kernel timing ---------------------------------------------------------
+ * The library-side counterpart of the reference's GPU profiler scopes
+ * (PROFILE("Network inference"), integrator.cpp:640-646; profiler.h:229-233):
+ * while enabled, HIP events bracket every launch of the tagged kernels on the
+ * stream it runs on. Tags: NPMG_TIME_GRID_ENCODE (level-major grid encoding
+ * of the MFMA path), NPMG_TIME_QUERY (the fused query kernel, FP32 or MFMA),
+ * NPMG_TIME_TRAIN_STEP (one whole npmg_train_step / npmg_train_step_graph). */
+#define NPMG_TIME_GRID_ENCODE 0u
+#define NPMG_TIME_QUERY 1u
+#define NPMG_TIME_TRAIN_STEP 2u
+npmg_status npmg_timing_enable(npmg_ctx* ctx, int enable);
+/* Waits for the recorded launches, returns their total time (ms) and count
+ * for one tag since the last read, and resets that tag. Host-synchronous. */
+npmg_status npmg_timing_read(npmg_ctx* ctx, uint32_t tag, double* total_ms, uint32_t* launches);
+
 /* ---- FP64 validation kernels ---------------------------------------------
  * No counterpart in the reference, which has no tests: these are the checks
  * its pdf (distribution.h:100-119) and loss adjoint (train.cu:69-124) are

@@ -183,6 +183,12 @@ struct npmg_ctx {
     hipGraphExec_t graph_exec = nullptr;
     const void* graph_key[6] = {};
     static constexpr int kPipe = 4;
+    // kernel timing: event pairs per tag, recorded while enabled, read by npmg_timing_read
+    static constexpr int kTags = 3;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> t_pending[kTags], t_free;
+    double t_ms[kTags] = {};
+    uint32_t t_count[kTags] = {};
     hipEvent_t ev_fork = nullptr, ev_chunk[kPipe] = {};
     // MFMA training: transposed weight fragments and the operand-type buffers
     void* bfrag_train = nullptr;
@@ -199,6 +205,12 @@ struct npmg_ctx {
 
     ~npmg_ctx() {
         drop_graph();
+        for (auto& v : t_pending)
+            for (auto& e : v) t_free.push_back(e);
+        for (auto& e : t_free) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         if (capture) (void)hipStreamDestroy(capture);
         if (aux) (void)hipStreamDestroy(aux);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -213,6 +225,39 @@ struct npmg_ctx {
     }
 
     bool half() const { return cfg.precision == PREC_BF16 || cfg.precision == PREC_FP16; }
+
+    // Brackets the launches enqueued by f on stream s with a timing event pair (tag).
+    template <class F>
+    void timed(int tag, hipStream_t s, F&& f) {
+        if (!timing) {
+            f();
+            return;
+        }
+        std::pair<hipEvent_t, hipEvent_t> e{};
+        if (!t_free.empty()) {
+            e = t_free.back();
+            t_free.pop_back();
+        } else {
+            hip_check(hipEventCreate(&e.first), "event");
+            hip_check(hipEventCreate(&e.second), "event");
+        }
+        hip_check(hipEventRecord(e.first, s), "event");
+        f();
+        hip_check(hipEventRecord(e.second, s), "event");
+        t_pending[tag].push_back(e);
+    }
+
+    void timing_collect(int tag) {
+        for (auto& e : t_pending[tag]) {
+            hip_check(hipEventSynchronize(e.second), "event sync");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, e.first, e.second), "event time");
+            t_ms[tag] += ms;
+            ++t_count[tag];
+            t_free.push_back(e);
+        }
+        t_pending[tag].clear();
+    }
 
     void check_fused() const {
         if (cfg.precision == PREC_FP8) throw std::invalid_argument("precision fp8 is not available in this build");
@@ -258,8 +303,10 @@ struct npmg_ctx {
             const void* frag = inference ? frag_infer : frag_train;
             const void* tab = inference ? htab_infer : htab_train;
             if (!mfma_split_grid(cfg.d)) {
-                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, s),
-                          "query (mfma)");
+                timed(NPMG_TIME_QUERY, s, [&] {
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, s),
+                              "query (mfma)");
+                });
                 return;
             }
             ensure_genc(n);
@@ -269,29 +316,40 @@ struct npmg_ctx {
             const uint32_t chunks = n >= (1u << 18) ? (uint32_t)kPipe : 1u;
             const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
             if (chunks == 1u) {
-                hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s), "grid encode");
-                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, s),
-                          "query (mfma)");
+                timed(NPMG_TIME_GRID_ENCODE, s, [&] {
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s),
+                              "grid encode");
+                });
+                timed(NPMG_TIME_QUERY, s, [&] {
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, s),
+                              "query (mfma)");
+                });
                 return;
             }
             hip_check(hipEventRecord(ev_fork, s), "event");
             hip_check(hipStreamWaitEvent(aux, ev_fork, 0), "stream wait");
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
-                hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux),
-                          "grid encode");
+                timed(NPMG_TIME_GRID_ENCODE, aux, [&] {
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux),
+                              "grid encode");
+                });
                 hip_check(hipEventRecord(ev_chunk[k], aux), "event");
             }
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 hip_check(hipStreamWaitEvent(s, ev_chunk[k], 0), "stream wait");
-                hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, s),
-                          "query (mfma)");
+                timed(NPMG_TIME_QUERY, s, [&] {
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, s),
+                              "query (mfma)");
+                });
             }
         } else {
             const float* p = inference ? infer_params() : master;
-            hip_check(launch_query_fp32(cfg.d, inference ? wt_infer : wt_train, p + cfg.n_mlp, io, n, n_cu, s),
-                      "query (fp32)");
+            timed(NPMG_TIME_QUERY, s, [&] {
+                hip_check(launch_query_fp32(cfg.d, inference ? wt_infer : wt_train, p + cfg.n_mlp, io, n, n_cu, s),
+                          "query (fp32)");
+            });
         }
     }
 
@@ -775,7 +833,26 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
     return guarded(ctx, [&] {
         ctx->check_train();
         need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
-        ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, S(stream));
+        ctx->timed(NPMG_TIME_TRAIN_STEP, S(stream),
+                   [&] { ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, S(stream)); });
+    });
+}
+
+npmg_status npmg_timing_enable(npmg_ctx* ctx, int enable) {
+    if (!ctx) return NPMG_EINVAL;
+    ctx->timing = enable != 0;
+    return NPMG_OK;
+}
+
+npmg_status npmg_timing_read(npmg_ctx* ctx, uint32_t tag, double* total_ms, uint32_t* launches) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(tag < (uint32_t)npmg_ctx::kTags, "unknown timing tag");
+        ctx->timing_collect((int)tag);
+        if (total_ms) *total_ms = ctx->t_ms[tag];
+        if (launches) *launches = ctx->t_count[tag];
+        ctx->t_ms[tag] = 0.0;
+        ctx->t_count[tag] = 0;
     });
 }
 
@@ -803,7 +880,8 @@ npmg_status npmg_train_step_graph(npmg_ctx* ctx, const float* in, const float* t
             hip_check(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
             std::copy(key, key + 6, ctx->graph_key);
         }
-        hip_check(hipGraphLaunch(ctx->graph_exec, S(stream)), "launch graph");
+        ctx->timed(NPMG_TIME_TRAIN_STEP, S(stream),
+                   [&] { hip_check(hipGraphLaunch(ctx->graph_exec, S(stream)), "launch graph"); });
     });
 }
 

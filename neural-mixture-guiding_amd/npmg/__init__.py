@@ -24,6 +24,7 @@ HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "npmg.
 
 NPMG_OK, NPMG_EINVAL, NPMG_ENOMEM, NPMG_EHIP, NPMG_ESTATE = range(5)
 NPMG_COSINE_AWARE = 1
+NPMG_TIME_GRID_ENCODE, NPMG_TIME_QUERY, NPMG_TIME_TRAIN_STEP = 0, 1, 2
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -57,6 +58,8 @@ SIGNATURES = {
     "npmg_synth_train": (_int, [_u64, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "npmg_debug_math": (_int, [_int, _vp, _vp, _u32, _vp, _vp]),
     "npmg_train_step_graph": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
+    "npmg_timing_enable": (_int, [_vp, _int]),
+    "npmg_timing_read": (_int, [_vp, _u32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
     "npmg_validate_pdf_integral": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp]),
     "npmg_validate_loss_fd": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp, _vp]),
 }
@@ -361,6 +364,15 @@ class GuidingNetwork:
         self._check(self._lib.npmg_train_step_graph(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb),
                                                      _ptr(tgt_pdf), n, _ptr(loss_out), _stream()))
         return loss_out
+
+    def timing(self, enable: bool):
+        self._check(self._lib.npmg_timing_enable(self._h, int(enable)))
+
+    def timing_read(self, tag: int):
+        """(total ms, launches) of a timing tag since the last read (synchronises)."""
+        ms, cnt = ctypes.c_double(), _u32()
+        self._check(self._lib.npmg_timing_read(self._h, tag, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
 
     def train_state(self):
         s, t = _f32(), _u32()
