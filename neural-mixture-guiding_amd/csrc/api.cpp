@@ -261,6 +261,7 @@ struct npmg_ctx {
 
     void check_fused() const {
         if (cfg.precision == PREC_FP8) throw std::invalid_argument("precision fp8 is not available in this build");
+        if (cfg.d.mixture == MIX_ASG && cfg.d.n_hidden < 1) throw std::invalid_argument("invalid network");
         if (cfg.d.K != 8 && cfg.d.K != 16) throw std::invalid_argument("the fused kernels support 8 or 16 lobes");
     }
 
@@ -735,6 +736,8 @@ npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal,
         need(!u || (out_dir && out_pdf), "sampling needs out_dir and out_pdf");
         need(n_eval == 0 || (eval_dirs && out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
         need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
+        need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
+             "the anisotropic mixture has no cosine product");
         QueryIO io{};
         io.in = in;
         io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
@@ -760,7 +763,10 @@ npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float
         need(!u || (out_dir && out_pdf), "sampling needs out_dir and out_pdf");
         need(n_eval == 0 || (eval_dirs && out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
         need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
+        need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
+             "the anisotropic mixture has no cosine product");
         need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the mixture kernel supports 8 or 16 lobes");
+        need(!lobes_out || ctx->cfg.d.mixture != MIX_ASG, "lobe parameters are written for vMF lobes only");
         QueryIO io{};
         io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
         io.u = u;
@@ -902,6 +908,7 @@ npmg_status npmg_validate_pdf_integral(npmg_ctx* ctx, const float* raw, const fl
     return guarded(ctx, [&] {
         need(n == 0 || (raw && out), "null buffer");
         need(n_z >= 2 && n_z <= 1u << 16 && n_phi >= 1 && n_phi <= 1u << 20, "quadrature size out of range");
+        need(!normal || ctx->cfg.d.mixture != MIX_ASG, "the anisotropic mixture has no cosine product");
         need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the validation kernels support 8 or 16 lobes");
         if (ctx->gl_n != n_z) {
             std::vector<double> r = gauss_legendre(n_z);

@@ -151,10 +151,13 @@ Config parse_config(const std::string& text) {
     const Json empty;
     const Json& mix = root.contains("mixture") ? root.at("mixture") : empty;
     const std::string mt = mix.kind == Json::Object ? mix.string("otype", "vMF") : "vMF";
-    if (!ieq(mt, "vMF")) bad("mixture otype must be vMF");
-    d.mixture = MIX_VMF;
-    d.K = mix.kind == Json::Object ? (int)mix.number("n_lobes", 8) : 8;
+    if (ieq(mt, "vMF")) d.mixture = MIX_VMF;
+    else if (ieq(mt, "ASG")) d.mixture = MIX_ASG;
+    else bad("mixture otype must be vMF or ASG");
+    d.K = mix.kind == Json::Object ? (int)mix.number("n_lobes", d.mixture == MIX_ASG ? 16 : 8) : 8;
     if (d.K < 1 || d.K > kMaxLobes) bad("n_lobes must be in [1, 16]");
+    if (d.mixture == MIX_ASG && d.K != 16) bad("the anisotropic mixture has 16 lobes");
+    d.lobe_params = d.mixture == MIX_ASG ? 6 : 4;
     const std::string mapping = mix.kind == Json::Object ? mix.string("mapping", "reference") : "reference";
     if (ieq(mapping, "reference")) d.mapping = MAP_REFERENCE;
     else if (ieq(mapping, "softmax")) d.mapping = MAP_SOFTMAX;
@@ -164,7 +167,7 @@ Config parse_config(const std::string& text) {
     else if (ieq(num, "stable")) d.numerics = NUM_STABLE;
     else bad("mixture numerics must be reference or stable");
     d.cos_kappa = mix.kind == Json::Object ? (float)mix.number("cosine_kappa", 2.18853) : 2.18853f;
-    d.n_out = 4 * d.K;
+    d.n_out = d.lobe_params * d.K;
     d.n_out_pad = (d.n_out + 15) / 16 * 16;
 
     // ---- network (tiny-cuda-nn networks/fully_fused_mlp.h) ----

@@ -14,7 +14,7 @@ enum HashType : int { HASH_COHERENT_PRIME = 0, HASH_PRIME = 1, HASH_REVERSED_PRI
 enum Interp : int { INTERP_LINEAR = 0, INTERP_SMOOTHSTEP = 1, INTERP_NEAREST = 2 };
 enum Mapping : int { MAP_REFERENCE = 0, MAP_SOFTMAX = 1 };
 enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1 };
-enum MixtureType : int { MIX_VMF = 0 };
+enum MixtureType : int { MIX_VMF = 0, MIX_ASG = 1 };
 enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3 };
 
 constexpr int kMaxEnc = 4;
@@ -37,6 +37,7 @@ struct DevCfg {
     int layer_in[kMaxLayers], layer_out[kMaxLayers];
     uint32_t layer_off[kMaxLayers];  // offset of each weight matrix in the parameter vector
     int K, mapping, numerics, mixture;
+    int lobe_params;  // raw outputs per lobe: 4 (vMF), 6 (anisotropic)
     float cos_kappa, grad_clamp;
     float clamp_scale;  // loss scale at which grad_clamp applies (threshold grad_clamp * scale / clamp_scale)
 };

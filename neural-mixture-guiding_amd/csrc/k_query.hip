@@ -9,6 +9,7 @@
 // lane-consecutive, so conflict-free). Encoded features are streamed straight
 // into the first layer as they are produced, so the encoding never touches
 // HBM. Accumulation order over inputs is ascending, as in the oracle.
+#include "asg.h"
 #include "device.h"
 #include "kernels.h"
 
@@ -76,10 +77,15 @@ __device__ __forceinline__ void mlp_forward(const DevCfg& c, const float* __rest
     }
 }
 
-// Mixture work of one query from its raw outputs.
-template <int K, int NOUT>
+// Mixture work of one query from its raw outputs (P raw outputs per lobe:
+// 4 vMF, 6 anisotropic).
+template <int K, int P, int NOUT>
 __device__ __forceinline__ void mixture_query(const DevCfg& c, const float* out, const QueryIO& io, uint32_t q,
                                               uint32_t n) {
+    if constexpr (P == 6) {
+        asg_query<K>([&](int j) { return out[j]; }, io, q, n);
+        return;
+    }
     Mixture<K> m;
     build_mixture<K>(c, [&](int j) { return out[j]; }, m);
     float th[K] = {}, ph[K] = {};
@@ -120,12 +126,12 @@ __device__ __forceinline__ void mixture_query(const DevCfg& c, const float* out,
     }
 }
 
-template <int W, int K, bool LDS_W>
+template <int W, int K, int P, bool LDS_W>
 __global__ void __launch_bounds__(block_size<W>()) k_query_fp32(const DevCfg c, const float* __restrict__ wt,
                                                                 const float* __restrict__ table, const QueryIO io,
                                                                 uint32_t n) {
     constexpr int BS = block_size<W>();
-    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
     extern __shared__ float smem[];
     const uint32_t n_mlp = n_mlp_of(c);
     const float* sw = wt;
@@ -149,20 +155,20 @@ __global__ void __launch_bounds__(block_size<W>()) k_query_fp32(const DevCfg c, 
 #pragma unroll
             for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
         }
-        if (io.u || io.n_eval > 0) mixture_query<K, NOUT>(c, out, io, q, n);
+        if (io.u || io.n_eval > 0) mixture_query<K, P, NOUT>(c, out, io, q, n);
     }
 }
 
-template <int K>
+template <int K, int P>
 __global__ void __launch_bounds__(256) k_mixture(const DevCfg c, const float* __restrict__ raw, const QueryIO io,
                                                  uint32_t n) {
-    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     if (q >= n) return;
     float out[NOUT];
 #pragma unroll
     for (int o = 0; o < NOUT; ++o) out[o] = o < c.n_out ? raw[(size_t)o * n + q] : 0.0f;
-    mixture_query<K, NOUT>(c, out, io, q, n);
+    mixture_query<K, P, NOUT>(c, out, io, q, n);
 }
 
 __global__ void __launch_bounds__(256) k_encode(const DevCfg c, const float* __restrict__ table,
@@ -193,18 +199,18 @@ __global__ void __launch_bounds__(256) k_grid_corners(const DevCfg c, const floa
     }
 }
 
-template <int W, int K, bool LDS_W>
+template <int W, int K, int P, bool LDS_W>
 hipError_t launch_query_t(const DevCfg& c, const float* wt, const float* table, const QueryIO& io, uint32_t n,
                           int n_cu, hipStream_t s) {
     constexpr int BS = block_size<W>();
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)W * BS * sizeof(float);
-    hipError_t e = hipFuncSetAttribute((const void*)k_query_fp32<W, K, LDS_W>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_query_fp32<W, K, P, LDS_W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const uint32_t tiles = (n + BS - 1) / BS;
     const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
-    hipLaunchKernelGGL((k_query_fp32<W, K, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, io, n);
+    hipLaunchKernelGGL((k_query_fp32<W, K, P, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, io, n);
     return hipGetLastError();
 }
 
@@ -220,22 +226,26 @@ hipError_t launch_query_fp32(const DevCfg& c, const float* wt, const float* tabl
                              int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const bool lds = fp32_weights_fit_lds(c);
-    if (c.width == 64 && c.K == 8) return lds ? launch_query_t<64, 8, true>(c, wt, table, io, n, n_cu, s)
-                                              : launch_query_t<64, 8, false>(c, wt, table, io, n, n_cu, s);
-    if (c.width == 64 && c.K == 16) return lds ? launch_query_t<64, 16, true>(c, wt, table, io, n, n_cu, s)
-                                               : launch_query_t<64, 16, false>(c, wt, table, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 8) return lds ? launch_query_t<128, 8, true>(c, wt, table, io, n, n_cu, s)
-                                               : launch_query_t<128, 8, false>(c, wt, table, io, n, n_cu, s);
-    if (c.width == 128 && c.K == 16) return lds ? launch_query_t<128, 16, true>(c, wt, table, io, n, n_cu, s)
-                                                : launch_query_t<128, 16, false>(c, wt, table, io, n, n_cu, s);
+#define NPMG_Q(WW, KK, PP)                                                           \
+    if (c.width == WW && c.K == KK && c.lobe_params == PP)                           \
+        return lds ? launch_query_t<WW, KK, PP, true>(c, wt, table, io, n, n_cu, s) \
+                   : launch_query_t<WW, KK, PP, false>(c, wt, table, io, n, n_cu, s);
+    NPMG_Q(64, 8, 4)
+    NPMG_Q(64, 16, 4)
+    NPMG_Q(128, 8, 4)
+    NPMG_Q(128, 16, 4)
+    NPMG_Q(64, 16, 6)
+    NPMG_Q(128, 16, 6)
+#undef NPMG_Q
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_mixture(const DevCfg& c, const float* raw, const QueryIO& io, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256);
-    if (c.K <= 8) hipLaunchKernelGGL((k_mixture<8>), grid, dim3(256), 0, s, c, raw, io, n);
-    else hipLaunchKernelGGL((k_mixture<16>), grid, dim3(256), 0, s, c, raw, io, n);
+    if (c.lobe_params == 6) hipLaunchKernelGGL((k_mixture<16, 6>), grid, dim3(256), 0, s, c, raw, io, n);
+    else if (c.K <= 8) hipLaunchKernelGGL((k_mixture<8, 4>), grid, dim3(256), 0, s, c, raw, io, n);
+    else hipLaunchKernelGGL((k_mixture<16, 4>), grid, dim3(256), 0, s, c, raw, io, n);
     return hipGetLastError();
 }
 

@@ -15,6 +15,7 @@
 // between lanes l and l ^ 32 and exchanged with one shuffle, after which
 // lane l owns every raw output of query l and runs the FP32 mixture code
 // (normaliser, softmax, log-sum-exp and sampling stay FP32).
+#include "asg.h"
 #include "device.h"
 #include "kernels.h"
 #include "mfma.h"
@@ -63,13 +64,14 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
 }
 
 // Queries [q_begin, q_end) of SoA arrays with row stride n.
-template <class T, int W, int K>
+template <class T, int W, int K, int P>
 __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __restrict__ wfrag,
                                                     const T* __restrict__ table, const T* __restrict__ genc,
                                                     const QueryIO io, uint32_t n, uint32_t n_frag, uint32_t q_begin,
                                                     uint32_t q_end) {
     using V = typename mf::Frag<T>::vec;
-    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
+    static_assert(NOUT % 32 == 0, "whole 32-row output tiles");
     constexpr int TO = NOUT / 32;  // output tiles
     constexpr int TH = W / 32;     // hidden tiles
     constexpr int KBH = W / 16;    // K-blocks of a hidden layer's input
@@ -174,7 +176,9 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const T* __r
 #pragma unroll
                 for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
             }
-            if (io.u || io.n_eval > 0) {
+            if constexpr (P == 6) {
+                if (io.u || io.n_eval > 0) asg_query<K>([&](int jj) { return out[jj]; }, io, q, n);
+            } else if (io.u || io.n_eval > 0) {
                 Mixture<K> m;
                 build_mixture<K>(c, [&](int jj) { return out[jj]; }, m);
                 float th[K], ph[K];
@@ -232,13 +236,13 @@ __global__ void __launch_bounds__(256) k_to_half(const float* __restrict__ src, 
     if (i < n) dst[i] = (T)src[i];
 }
 
-template <class T, int W, int K>
+template <class T, int W, int K, int P>
 hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const QueryIO& io,
                          uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, hipStream_t s) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const uint32_t n_frag = n_mlp / 512u;
     const size_t lds = (size_t)n_frag * 1024u;
-    auto kern = k_query_mfma<T, W, K>;
+    auto kern = k_query_mfma<T, W, K, P>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
@@ -256,10 +260,16 @@ hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, 
 template <class T>
 hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* table, const void* genc,
                             const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, hipStream_t s) {
-    if (c.width == 64 && c.K == 8) return launch_mfma_t<T, 64, 8>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
-    if (c.width == 64 && c.K == 16) return launch_mfma_t<T, 64, 16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
-    if (c.width == 128 && c.K == 8) return launch_mfma_t<T, 128, 8>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
-    if (c.width == 128 && c.K == 16) return launch_mfma_t<T, 128, 16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+#define NPMG_QM(WW, KK, PP)                                      \
+    if (c.width == WW && c.K == KK && c.lobe_params == PP)       \
+        return launch_mfma_t<T, WW, KK, PP>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, s);
+    NPMG_QM(64, 8, 4)
+    NPMG_QM(64, 16, 4)
+    NPMG_QM(128, 8, 4)
+    NPMG_QM(128, 16, 4)
+    NPMG_QM(64, 16, 6)
+    NPMG_QM(128, 16, 6)
+#undef NPMG_QM
     return hipErrorInvalidValue;
 }
 

@@ -2,6 +2,7 @@
 // fused mixture adjoint, MLP/grid backward, weight-gradient GEMM with a
 // deterministic split-K reduction, Adam + EMA, weight transposition and the
 // loss mean.
+#include "asg.h"
 #include "device.h"
 #include "kernels.h"
 
@@ -34,7 +35,7 @@ __device__ __forceinline__ void axpy_row(const float* __restrict__ w, float v, f
 // Forward for training: encoded features and post-ReLU activations are kept
 // in HBM for the weight gradients; the loss and dL/draw of the mixture-KL
 // adjoint are computed in the same thread (train.cu:69-124).
-template <int W, int K, bool LDS_W>
+template <int W, int K, int P, bool LDS_W>
 __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, const float* __restrict__ wt,
                                                                const float* __restrict__ table,
                                                                const float* __restrict__ in,
@@ -43,7 +44,7 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
                                                                const float* __restrict__ tgt_pdf, const LossScale ls,
                                                                const TrainWS ws, uint32_t n) {
     constexpr int BS = block_size<W>();
-    constexpr int NOUT = (4 * K + 15) / 16 * 16;
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
     extern __shared__ float smem[];
     const uint32_t n_mlp = n_mlp_of(c);
     const float* sw = wt;
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
             }
         }
         if (tgt_dir) {
-            const float loss = mixture_adjoint<K>(
+            const float loss = lobe_adjoint<K, P>(
                 c, [&](int jj) { return out[jj]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
                 tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
                 [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
@@ -100,14 +101,14 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
 }
 
 // Standalone adjoint from raw outputs (harness split pass).
-template <int K>
+template <int K, int P>
 __global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __restrict__ raw,
                                                  const float* __restrict__ tgt_dir, const float* __restrict__ tgt_rgb,
                                                  const float* __restrict__ tgt_pdf, const LossScale ls,
                                                  float* __restrict__ dout, float* __restrict__ loss, uint32_t n) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     if (q >= n) return;
-    loss[q] = mixture_adjoint<K>(
+    loss[q] = lobe_adjoint<K, P>(
         c, [&](int j) { return raw[(size_t)j * n + q]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
         tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
         [&](int j, float v) { dout[(size_t)j * n + q] = v; });
@@ -350,19 +351,19 @@ __global__ void __launch_bounds__(256) k_mean_final(const float* __restrict__ pa
     if (threadIdx.x == 0) out[0] = n ? s / (float)n : 0.0f;
 }
 
-template <int W, int K, bool LDS_W>
+template <int W, int K, int P, bool LDS_W>
 hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, const float* in, const float* td,
                         const float* tr, const float* tp, LossScale ls, const TrainWS& ws, uint32_t n, int n_cu,
                         hipStream_t s) {
     constexpr int BS = block_size<W>();
     const uint32_t n_mlp = n_mlp_of(c);
     const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)W * BS * sizeof(float);
-    hipError_t e = hipFuncSetAttribute((const void*)k_train_fwd<W, K, LDS_W>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_train_fwd<W, K, P, LDS_W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const uint32_t tiles = (n + BS - 1) / BS;
     const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
-    hipLaunchKernelGGL((k_train_fwd<W, K, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, in, td, tr, tp, ls,
+    hipLaunchKernelGGL((k_train_fwd<W, K, P, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, in, td, tr, tp, ls,
                        ws, n);
     return hipGetLastError();
 }
@@ -392,12 +393,17 @@ hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const flo
                                      uint32_t n, int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const bool lds = fp32_weights_fit_lds(c);
-    if (c.width == 64 && c.K == 8) return lds ? launch_fwd_t<64, 8, true>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s)
-                                              : launch_fwd_t<64, 8, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
-    if (c.width == 64 && c.K == 16) return lds ? launch_fwd_t<64, 16, true>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s)
-                                               : launch_fwd_t<64, 16, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
-    if (c.width == 128 && c.K == 8) return launch_fwd_t<128, 8, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
-    if (c.width == 128 && c.K == 16) return launch_fwd_t<128, 16, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+#define NPMG_F(WW, KK, PP)                                                                       \
+    if (c.width == WW && c.K == KK && c.lobe_params == PP)                                       \
+        return lds ? launch_fwd_t<WW, KK, PP, WW == 64>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s) \
+                   : launch_fwd_t<WW, KK, PP, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+    NPMG_F(64, 8, 4)
+    NPMG_F(64, 16, 4)
+    NPMG_F(128, 8, 4)
+    NPMG_F(128, 16, 4)
+    NPMG_F(64, 16, 6)
+    NPMG_F(128, 16, 6)
+#undef NPMG_F
     return hipErrorInvalidValue;
 }
 
@@ -411,6 +417,9 @@ hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const Tra
                                                        : launch_bwd_t<64, 64, false>(c, w, ws, denc, dgrid, n, n_cu, s);
     if (c.width == 128 && c.n_out_pad == 32) return launch_bwd_t<128, 32, false>(c, w, ws, denc, dgrid, n, n_cu, s);
     if (c.width == 128 && c.n_out_pad == 64) return launch_bwd_t<128, 64, false>(c, w, ws, denc, dgrid, n, n_cu, s);
+    if (c.width == 64 && c.n_out_pad == 96) return lds ? launch_bwd_t<64, 96, true>(c, w, ws, denc, dgrid, n, n_cu, s)
+                                                       : launch_bwd_t<64, 96, false>(c, w, ws, denc, dgrid, n, n_cu, s);
+    if (c.width == 128 && c.n_out_pad == 96) return launch_bwd_t<128, 96, false>(c, w, ws, denc, dgrid, n, n_cu, s);
     return hipErrorInvalidValue;
 }
 
@@ -481,8 +490,9 @@ hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* td, co
                           LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256);
-    if (c.K == 8) hipLaunchKernelGGL((k_adjoint<8>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
-    else if (c.K == 16) hipLaunchKernelGGL((k_adjoint<16>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    if (c.lobe_params == 6) hipLaunchKernelGGL((k_adjoint<16, 6>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    else if (c.K == 8) hipLaunchKernelGGL((k_adjoint<8, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    else if (c.K == 16) hipLaunchKernelGGL((k_adjoint<16, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

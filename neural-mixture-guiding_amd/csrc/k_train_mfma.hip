@@ -464,11 +464,13 @@ hipError_t launch_reduce_splits(const float* partials, uint32_t n_splits, uint32
 bool mfma_split_grid(const DevCfg& c);
 
 // Shapes with compiled MFMA training kernels: width 64 with 2-4 hidden layers or
-// width 128 with 2-3, and 32 or 64 raw outputs.
+// width 128 with 2-3, and 32 or 64 raw outputs; 96 raw outputs (the
+// anisotropic mixture) with 3 hidden layers.
 bool mfma_train_supported(const DevCfg& c) {
-    const bool shape = (c.width == 64 && c.n_hidden >= 2 && c.n_hidden <= 4) ||
-                       (c.width == 128 && c.n_hidden >= 2 && c.n_hidden <= 3);
-    return shape && (c.n_out_pad == 32 || c.n_out_pad == 64) && c.enc_padded % 16 == 0;
+    const bool shape = c.n_out_pad == 96 ? ((c.width == 64 || c.width == 128) && c.n_hidden == 3)
+                                         : (c.width == 64 && c.n_hidden >= 2 && c.n_hidden <= 4) ||
+                                               (c.width == 128 && c.n_hidden >= 2 && c.n_hidden <= 3);
+    return shape && (c.n_out_pad == 32 || c.n_out_pad == 64 || c.n_out_pad == 96) && c.enc_padded % 16 == 0;
 }
 
 #define NPMG_TRAIN_SHAPES(CASE, TT) \
@@ -481,7 +483,9 @@ bool mfma_train_supported(const DevCfg& c) {
     CASE(TT, 128, 32, 2)            \
     CASE(TT, 128, 32, 3)            \
     CASE(TT, 128, 64, 2)            \
-    CASE(TT, 128, 64, 3)
+    CASE(TT, 128, 64, 3)            \
+    CASE(TT, 64, 96, 3)             \
+    CASE(TT, 128, 96, 3)
 
 hipError_t launch_train_forward_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
                                      const float* in, const TrainMfmaWS& ws, uint32_t n, int n_cu, hipStream_t s) {
