@@ -120,6 +120,42 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
 npmg_status npmg_train_step_graph(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
                                   const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream);
 
+/* One queue of a wavefront frame (e.g. one bounce's scatter queue), SoA with
+ * row stride `capacity`: in [9][capacity]; normal [3][capacity] (NULL: no
+ * cosine product); u [3][capacity] (NULL: no sampling); eval_dirs
+ * [n_eval][3][capacity]; outputs out_dir [3][capacity], out_pdf, out_lobe
+ * [capacity] (out_lobe may be NULL), out_eval_pdf [n_eval][capacity]. */
+typedef struct npmg_queue {
+    const float* in;
+    const float* normal;
+    const float* u;
+    const float* eval_dirs;
+    float* out_dir;
+    float* out_pdf;
+    float* out_eval_pdf;
+    uint32_t* out_lobe;
+    uint32_t capacity;
+} npmg_queue;
+
+/* npmg_sample_pdf over n_queues (<= 16) queues of uneven sizes in one
+ * persistent launch: workgroups pull 64-query tiles from a device work
+ * counter across all queues, so no queue's tail idles the GPU. counts: device
+ * array [n_queues] of each queue's length (read on the device, as the
+ * wavefront queues' sizes are; values above capacity are clamped). Replaces
+ * the per-queue launches over MAX_INFERENCE_NUM threads of the guided passes
+ * (integrator.cpp:160-320, 627-648). queues: host array. */
+npmg_status npmg_sample_pdf_queues(npmg_ctx* ctx, const npmg_queue* queues, const uint32_t* counts, uint32_t n_queues,
+                                   uint32_t n_eval, uint32_t flags, npmg_stream stream);
+
+/* Picks the launch shape of the query path for batches of about n queries by
+ * timing the candidates on synthetic queries (pipeline chunks 1/2/4/8 of the
+ * BF16/FP16 path and workgroups per CU of the fused and persistent kernels),
+ * and keeps the fastest for later calls. Synchronous; allocates scratch. */
+npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream);
+/* The launch shape in use: pipeline chunks and workgroups per CU (0 = as
+ * many as fit). */
+npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_per_cu);
+
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 
 /* Encoding only: in [9][n] -> enc [encoded_width][n]. */

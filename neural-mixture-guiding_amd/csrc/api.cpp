@@ -28,7 +28,13 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
 hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const void* genc,
                              const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu,
-                             hipStream_t s);
+                             int blocks_per_cu, hipStream_t s);
+hipError_t launch_query_mfma_queues(const DevCfg& c, int precision, const void* wfrag, const void* table,
+                                    const QueueSet& qs, int n_cu, int blocks_per_cu, hipStream_t s);
+hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void* table, const QueueSet& qs, int n_cu,
+                                     hipStream_t s);
+hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
+                                    int n_cu, hipStream_t s);
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
                                     uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s);
 bool mfma_split_grid(const DevCfg& c);
@@ -147,6 +153,10 @@ void init_params(const Config& cfg, uint32_t seed, std::vector<float>& p) {
     }
 }
 
+void need(bool ok, const char* msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
 }  // namespace
 
 struct npmg_ctx {
@@ -182,7 +192,11 @@ struct npmg_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     const void* graph_key[6] = {};
-    static constexpr int kPipe = 4;
+    static constexpr int kPipe = 8;
+    // launch shape of the query path (npmg_autotune): pipeline chunks, workgroups per CU (0: all that fit)
+    uint32_t tune_chunks = 4;
+    int tune_bpc = 0;
+    uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
     // kernel timing: event pairs per tag, recorded while enabled, read by npmg_timing_read
     static constexpr int kTags = 3;
     bool timing = false;
@@ -220,7 +234,8 @@ struct npmg_ctx {
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
-                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes})
+                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes,
+                        (void*)queue_counter})
             if (p) (void)hipFree(p);
     }
 
@@ -305,16 +320,16 @@ struct npmg_ctx {
             const void* tab = inference ? htab_infer : htab_train;
             if (!mfma_split_grid(cfg.d)) {
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, tune_bpc, s),
                               "query (mfma)");
                 });
                 return;
             }
             ensure_genc(n);
-            // Large batches: kPipe chunks; the encodes run on aux (after everything
+            // Large batches: tune_chunks (<= kPipe) chunks; the encodes run on aux (after everything
             // already queued on s, which may still read genc), chunk c's fused
             // kernel on s after chunk c's encode.
-            const uint32_t chunks = n >= (1u << 18) ? (uint32_t)kPipe : 1u;
+            const uint32_t chunks = n >= (1u << 18) ? tune_chunks : 1u;
             const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
             if (chunks == 1u) {
                 timed(NPMG_TIME_GRID_ENCODE, s, [&] {
@@ -322,7 +337,7 @@ struct npmg_ctx {
                               "grid encode");
                 });
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, tune_bpc, s),
                               "query (mfma)");
                 });
                 return;
@@ -341,7 +356,7 @@ struct npmg_ctx {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 hip_check(hipStreamWaitEvent(s, ev_chunk[k], 0), "stream wait");
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, tune_bpc, s),
                               "query (mfma)");
                 });
             }
@@ -352,6 +367,125 @@ struct npmg_ctx {
                           "query (fp32)");
             });
         }
+    }
+
+    // Several queues in one persistent launch (inference parameters): the grid
+    // encode pulls (level, tile) items over all queues, then the fused kernel
+    // pulls 64-query tiles; each launch has its own work counter.
+    void query_queues(const npmg_queue* qv, const uint32_t* counts, uint32_t nq, uint32_t n_eval, uint32_t flags,
+                      hipStream_t s) {
+        QueueSet qs{};
+        qs.n_queues = nq;
+        qs.counts = counts;
+        uint64_t total_cap = 0;
+        for (uint32_t i = 0; i < nq; ++i) {
+            const npmg_queue& q = qv[i];
+            need(q.capacity == 0 || q.in, "null queue input");
+            need(!q.u || (q.out_dir && q.out_pdf), "sampling needs out_dir and out_pdf");
+            need(n_eval == 0 || (q.eval_dirs && q.out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
+            need(!(flags & NPMG_COSINE_AWARE) || q.normal, "NPMG_COSINE_AWARE needs normal");
+            QueryIO& io = qs.io[i];
+            io.in = q.in;
+            io.normal = (flags & NPMG_COSINE_AWARE) ? q.normal : nullptr;
+            io.u = q.u;
+            io.eval_dirs = q.eval_dirs;
+            io.n_eval = (int)n_eval;
+            io.out_dir = q.out_dir;
+            io.out_pdf = q.out_pdf;
+            io.out_eval_pdf = q.out_eval_pdf;
+            io.out_lobe = q.out_lobe;
+            qs.stride[i] = q.capacity;
+            total_cap += q.capacity;
+        }
+        need(total_cap < (1ull << 31), "total queue capacity must stay below 2^31");
+        if (total_cap == 0) return;
+        hip_check(launch_zero_u32(queue_counter, 2u, s), "zero");
+        timed(NPMG_TIME_QUERY, s, [&] {
+            if (half()) {
+                const bool split = mfma_split_grid(cfg.d);
+                if (split) {
+                    ensure_genc((uint32_t)total_cap);
+                    const size_t per_q = (size_t)cfg.d.n_levels * cfg.d.F;
+                    size_t off = 0;
+                    for (uint32_t i = 0; i < nq; ++i) {
+                        qs.genc[i] = static_cast<uint16_t*>(genc) + per_q * off;
+                        off += qv[i].capacity;
+                    }
+                    qs.counter = queue_counter;
+                    hip_check(launch_grid_encode_queues(cfg.d, cfg.precision, htab_infer, qs, n_cu, s),
+                              "grid encode (queues)");
+                }
+                qs.counter = queue_counter + 1;
+                hip_check(launch_query_mfma_queues(cfg.d, cfg.precision, frag_infer, htab_infer, qs, n_cu, tune_bpc, s),
+                          "query (mfma, queues)");
+            } else {
+                qs.counter = queue_counter + 1;
+                hip_check(launch_query_fp32_queues(cfg.d, wt_infer, infer_params() + cfg.n_mlp, qs, n_cu, s),
+                          "query (fp32, queues)");
+            }
+        });
+    }
+
+    // Times the launch shapes of the query path on n synthetic queries and keeps
+    // the fastest: pipeline chunks (split-grid MFMA path) x workgroups per CU.
+    void autotune(uint32_t n, hipStream_t s) {
+        if (!half() || n == 0) return;
+        const bool split = mfma_split_grid(cfg.d);
+        // synthetic queries: in [9][n], normal [3][n], u [3][n], eval_dirs [4][3][n]; outputs dir [3][n], pdf [n]
+        size_t bytes = sizeof(float) * (size_t)n * (9 + 3 + 3 + 12 + 3 + 1);
+        float* buf = nullptr;
+        hip_check(hipMalloc((void**)&buf, bytes), "hipMalloc");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            float *in = buf, *nrm = in + 9 * (size_t)n, *u = nrm + 3 * (size_t)n, *ed = u + 3 * (size_t)n;
+            float *od = ed + 12 * (size_t)n, *op = od + 3 * (size_t)n;
+            hip_check(launch_synth_queries(0x7a11u, 0u, n, in, nrm, u, ed, s), "synth");
+            QueryIO io{};
+            io.in = in;
+            io.normal = cfg.d.mixture == MIX_ASG ? nullptr : nrm;
+            io.u = u;
+            io.out_dir = od;
+            io.out_pdf = op;
+            hip_check(hipEventCreate(&e0), "event");
+            hip_check(hipEventCreate(&e1), "event");
+            const bool was_timing = timing;
+            timing = false;
+            float best = 1e30f;
+            uint32_t best_c = tune_chunks;
+            int best_b = tune_bpc;
+            for (uint32_t ch : {1u, 2u, 4u, 8u}) {
+                if (!split && ch > 1u) continue;
+                if (split && n < (1u << 18) && ch > 1u) continue;  // query() does not pipeline these
+                for (int b : {0, 1, 2, 3}) {
+                    tune_chunks = ch;
+                    tune_bpc = b;
+                    query(io, n, true, s);  // warm-up
+                    hip_check(hipEventRecord(e0, s), "event");
+                    for (int r = 0; r < 3; ++r) query(io, n, true, s);
+                    hip_check(hipEventRecord(e1, s), "event");
+                    hip_check(hipEventSynchronize(e1), "event");
+                    float ms = 0.0f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+                    if (ms < best * 0.98f) {  // prefer the earlier (simpler) shape unless clearly faster
+                        best = ms;
+                        best_c = ch;
+                        best_b = b;
+                    }
+                }
+            }
+            timing = was_timing;
+            tune_chunks = best_c;
+            tune_bpc = best_b;
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipFree(buf);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hip_check(hipFree(buf), "hipFree");
     }
 
     void free_ws() {
@@ -568,10 +702,6 @@ std::vector<double> gauss_legendre(uint32_t n) {
     return r;
 }
 
-void need(bool ok, const char* msg) {
-    if (!ok) throw std::invalid_argument(msg);
-}
-
 }  // namespace
 
 extern "C" {
@@ -615,6 +745,7 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         }
         ctx->sstate = dalloc<ScaleState>(1);
         ctx->mean_partials = dalloc<float>(256);
+        ctx->queue_counter = dalloc<uint32_t>(2);
         const ScaleState st0{c.loss_scale, 0u, 0u, 0u};
         hip_check(hipMemcpy(ctx->sstate, &st0, sizeof(st0), hipMemcpyHostToDevice), "upload");
         // decay^t for the EMA debiasing (tcnn ema.h:110-111 evaluates std::pow in
@@ -751,6 +882,34 @@ npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal,
         io.raw_out = raw_out;
         ctx->query(io, n, true, S(stream));
     });
+}
+
+npmg_status npmg_sample_pdf_queues(npmg_ctx* ctx, const npmg_queue* queues, const uint32_t* counts, uint32_t n_queues,
+                                   uint32_t n_eval, uint32_t flags, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n_queues <= (uint32_t)kMaxQueues, "at most 16 queues per launch");
+        need(n_queues == 0 || (queues && counts), "null queues or counts");
+        need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
+             "the anisotropic mixture has no cosine product");
+        if (n_queues) ctx->query_queues(queues, counts, n_queues, n_eval, flags, S(stream));
+    });
+}
+
+npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        ctx->autotune(n, S(stream));
+    });
+}
+
+npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_per_cu) {
+    if (!ctx || !chunks || !blocks_per_cu) return NPMG_EINVAL;
+    *chunks = ctx->half() && mfma_split_grid(ctx->cfg.d) ? ctx->tune_chunks : 1u;
+    *blocks_per_cu = (uint32_t)ctx->tune_bpc;
+    return NPMG_OK;
 }
 
 npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float* normal, const float* u,

@@ -21,7 +21,7 @@ using namespace dev;
 template <int W>
 constexpr int block_size() { return W == 64 ? 256 : 128; }
 
-__device__ __forceinline__ uint32_t n_mlp_of(const DevCfg& c) {
+__host__ __device__ __forceinline__ uint32_t n_mlp_of(const DevCfg& c) {
     const int L = c.n_layers - 1;
     return c.layer_off[L] + (uint32_t)(c.layer_in[L] * c.layer_out[L]);
 }
@@ -126,13 +126,10 @@ __device__ __forceinline__ void mixture_query(const DevCfg& c, const float* out,
     }
 }
 
-template <int W, int K, int P, bool LDS_W>
-__global__ void __launch_bounds__(block_size<W>()) k_query_fp32(const DevCfg c, const float* __restrict__ wt,
-                                                                const float* __restrict__ table, const QueryIO io,
-                                                                uint32_t n) {
-    constexpr int BS = block_size<W>();
-    constexpr int NOUT = (P * K + 15) / 16 * 16;
-    extern __shared__ float smem[];
+// Weights to LDS when they fit; returns the weights' base and this thread's activation column.
+template <int BS, bool LDS_W>
+__device__ __forceinline__ const float* stage_weights(const DevCfg& c, const float* __restrict__ wt, float* smem,
+                                                      float** act) {
     const uint32_t n_mlp = n_mlp_of(c);
     const float* sw = wt;
     float* act_base = smem;
@@ -142,20 +139,72 @@ __global__ void __launch_bounds__(block_size<W>()) k_query_fp32(const DevCfg c, 
         sw = smem;
         act_base = smem + ((n_mlp + 3u) & ~3u);
     }
-    float* act = act_base + threadIdx.x;
+    *act = act_base + threadIdx.x;
+    return sw;
+}
+
+// One query q of a stream with row stride n.
+template <int W, int K, int P, int BS>
+__device__ __forceinline__ void query_one_fp32(const DevCfg& c, const float* sw, const float* __restrict__ table,
+                                               float* act, const QueryIO& io, uint32_t n, uint32_t q) {
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
+    float qin[16];
+#pragma unroll
+    for (int d = 0; d < 16; ++d) qin[d] = d < c.n_in ? io.in[(size_t)d * n + q] : 0.0f;
+    float out[NOUT];
+    mlp_forward<W, NOUT, BS>(c, sw, table, qin, act, out, nullptr, nullptr, q, n);
+    if (io.raw_out) {
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
+    }
+    if (io.u || io.n_eval > 0) mixture_query<K, P, NOUT>(c, out, io, q, n);
+}
+
+template <int W, int K, int P, bool LDS_W>
+__global__ void __launch_bounds__(block_size<W>()) k_query_fp32(const DevCfg c, const float* __restrict__ wt,
+                                                                const float* __restrict__ table, const QueryIO io,
+                                                                uint32_t n) {
+    constexpr int BS = block_size<W>();
+    extern __shared__ float smem[];
+    float* act;
+    const float* sw = stage_weights<BS, LDS_W>(c, wt, smem, &act);
     for (uint32_t tile = blockIdx.x; (size_t)tile * BS < n; tile += gridDim.x) {
         const uint32_t q = tile * BS + threadIdx.x;
-        if (q >= n) continue;
-        float qin[16];
+        if (q < n) query_one_fp32<W, K, P, BS>(c, sw, table, act, io, n, q);
+    }
+}
+
+// Several queues in one persistent launch: workgroups pull tiles of BS
+// queries from a device counter (see k_query_mfma_queues).
+template <int W, int K, int P, bool LDS_W>
+__global__ void __launch_bounds__(block_size<W>()) k_query_fp32_queues(const DevCfg c, const float* __restrict__ wt,
+                                                                       const float* __restrict__ table,
+                                                                       const QueueSet qs) {
+    constexpr int BS = block_size<W>();
+    extern __shared__ float smem[];
+    __shared__ uint32_t item;
+    float* act;
+    const float* sw = stage_weights<BS, LDS_W>(c, wt, smem, &act);
+    uint32_t first[kMaxQueues + 1];
+    queue_tile_prefix(qs, (uint32_t)BS, first);
+    const uint32_t total = first[kMaxQueues];
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(qs.counter, 1u);
+        __syncthreads();
+        const uint32_t w = item;
+        __syncthreads();
+        if (w >= total) break;
+        int qi = 0;
+        uint32_t base = 0;
 #pragma unroll
-        for (int d = 0; d < 16; ++d) qin[d] = d < c.n_in ? io.in[(size_t)d * n + q] : 0.0f;
-        float out[NOUT];
-        mlp_forward<W, NOUT, BS>(c, sw, table, qin, act, out, nullptr, nullptr, q, n);
-        if (io.raw_out) {
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
-        }
-        if (io.u || io.n_eval > 0) mixture_query<K, P, NOUT>(c, out, io, q, n);
+        for (int i = 1; i < kMaxQueues; ++i)
+            if (w >= first[i] && first[i] < first[i + 1]) {
+                qi = i;
+                base = first[i];
+            }
+        const uint32_t q = (w - base) * BS + threadIdx.x;
+        if (q < min(qs.counts[qi], qs.stride[qi]))
+            query_one_fp32<W, K, P, BS>(c, sw, table, act, qs.io[qi], qs.stride[qi], q);
     }
 }
 
@@ -237,6 +286,39 @@ hipError_t launch_query_fp32(const DevCfg& c, const float* wt, const float* tabl
     NPMG_Q(64, 16, 6)
     NPMG_Q(128, 16, 6)
 #undef NPMG_Q
+    return hipErrorInvalidValue;
+}
+
+template <int W, int K, int P, bool LDS_W>
+hipError_t launch_queues_t(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs, int n_cu,
+                           hipStream_t s) {
+    constexpr int BS = block_size<W>();
+    const uint32_t n_mlp = n_mlp_of(c);
+    const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)W * BS * sizeof(float);
+    auto kern = k_query_fp32_queues<W, K, P, LDS_W>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((uint32_t)n_cu), dim3(BS), lds, s, c, wt, table, qs);
+    return hipGetLastError();
+}
+
+// Persistent FP32 launch over several queues (qs.counter must be zero).
+hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
+                                    int n_cu, hipStream_t s) {
+    if (qs.n_queues == 0) return hipSuccess;
+    if (qs.n_queues > (uint32_t)kMaxQueues) return hipErrorInvalidValue;
+    const bool lds = fp32_weights_fit_lds(c);
+#define NPMG_QF(WW, KK, PP)                                                  \
+    if (c.width == WW && c.K == KK && c.lobe_params == PP)                   \
+        return lds ? launch_queues_t<WW, KK, PP, true>(c, wt, table, qs, n_cu, s) \
+                   : launch_queues_t<WW, KK, PP, false>(c, wt, table, qs, n_cu, s);
+    NPMG_QF(64, 8, 4)
+    NPMG_QF(64, 16, 4)
+    NPMG_QF(128, 8, 4)
+    NPMG_QF(128, 16, 4)
+    NPMG_QF(64, 16, 6)
+    NPMG_QF(128, 16, 6)
+#undef NPMG_QF
     return hipErrorInvalidValue;
 }
 

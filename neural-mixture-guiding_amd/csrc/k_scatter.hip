@@ -375,6 +375,12 @@ Scratch carve(const DevCfg& c, const BinLayout& b, uint32_t n_bins, uint32_t n, 
 
 }  // namespace
 
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255u) / 256u), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic) {
     if (c.grid_enc < 0 || n == 0) return 0;
     uint32_t n_bins;

@@ -23,6 +23,28 @@ struct QueryIO {
     float* lobes_out;        // [K][5][n] or null (consumer kernel only)
 };
 
+// Several query streams (the per-bounce queues of a wavefront frame) served by
+// one persistent launch: the query counts live on the device, everything
+// else is per queue.
+constexpr int kMaxQueues = 16;
+struct QueueSet {
+    QueryIO io[kMaxQueues];
+    uint32_t stride[kMaxQueues];  // row stride (capacity) of each queue's arrays
+    void* genc[kMaxQueues];       // level-major grid features of each queue (MFMA path)
+    const uint32_t* counts;       // [n_queues] queries per queue (device)
+    uint32_t* counter;            // work counter (device), zero before the launch
+    uint32_t n_queues;
+};
+
+// first[i] = tiles of `tile` queries in queues 0 .. i-1; first[n_queues] = total.
+__device__ __forceinline__ void queue_tile_prefix(const QueueSet& qs, uint32_t tile, uint32_t* first) {
+    first[0] = 0u;
+    for (uint32_t i = 0; i < (uint32_t)kMaxQueues; ++i) {
+        const uint32_t t = i < qs.n_queues ? (min(qs.counts[i], qs.stride[i]) + tile - 1u) / tile : 0u;
+        first[i + 1] = first[i] + t;
+    }
+}
+
 // Training-side activations kept for the backward pass (SoA, device).
 struct TrainWS {
     float* enc;   // [enc_padded][n]
@@ -91,6 +113,8 @@ hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const Tra
 // grid_scatter_bytes; its first grid_scatter_clear_bytes must be zero before
 // the first call (the scatter leaves them zero again).
 size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic);
+// Zero fill by a kernel (safe inside captured graphs, unlike a memset node).
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic);
 hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad, uint32_t n,
                                bool deterministic, void* scratch, size_t scratch_bytes, hipStream_t s);

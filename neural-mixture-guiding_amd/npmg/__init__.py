@@ -33,6 +33,12 @@ _f32 = ctypes.c_float
 _int = ctypes.c_int
 _sz = ctypes.c_size_t
 
+class NpmgQueue(ctypes.Structure):
+    """npmg_queue of include/npmg.h: one queue's SoA buffers, row stride `capacity`."""
+    _fields_ = [("in_", _vp), ("normal", _vp), ("u", _vp), ("eval_dirs", _vp), ("out_dir", _vp),
+                ("out_pdf", _vp), ("out_eval_pdf", _vp), ("out_lobe", _vp), ("capacity", _u32)]
+
+
 # name -> (restype, argtypes); every function declared in include/npmg.h.
 SIGNATURES = {
     "npmg_create": (_int, [ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
@@ -62,6 +68,9 @@ SIGNATURES = {
     "npmg_timing_read": (_int, [_vp, _u32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
     "npmg_validate_pdf_integral": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp]),
     "npmg_validate_loss_fd": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp, _vp]),
+    "npmg_sample_pdf_queues": (_int, [_vp, ctypes.POINTER(NpmgQueue), _vp, _u32, _u32, _u32, _vp]),
+    "npmg_autotune": (_int, [_vp, _u32, _vp]),
+    "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 }
 
 
@@ -357,6 +366,46 @@ class GuidingNetwork:
         self._check(self._lib.npmg_validate_loss_fd(self._h, _ptr(raw_out), _ptr(tgt_dir), _ptr(tgt_rgb),
                                                     _ptr(tgt_pdf), n, float(h), _ptr(out), _stream()))
         return out
+
+    def sample_pdf_queues(self, queues, counts, n_eval: int = 0, cosine_aware: bool = True):
+        """sample_pdf over several queues in one persistent launch.
+
+        queues: list of dicts with "inputs" [9][cap] and optionally "normal",
+        "u" [3][cap], "eval_dirs" [n_eval][3][cap]; counts: device uint32/int32
+        tensor [len(queues)] of live queries per queue. Returns one output dict
+        per queue (rows of stride cap; entries past the count are untouched).
+        """
+        import torch
+
+        nq = len(queues)
+        flags = NPMG_COSINE_AWARE if cosine_aware and nq and queues[0].get("normal") is not None else 0
+        arr = (NpmgQueue * max(nq, 1))()
+        outs = []
+        for i, q in enumerate(queues):
+            cap = q["inputs"].shape[1]
+            o = {}
+            if q.get("u") is not None:
+                o["dir"] = torch.zeros(3, cap, dtype=torch.float32, device="cuda")
+                o["pdf"] = torch.zeros(cap, dtype=torch.float32, device="cuda")
+                o["lobe"] = torch.zeros(cap, dtype=torch.int32, device="cuda")
+            if n_eval:
+                o["eval_pdf"] = torch.zeros(n_eval, cap, dtype=torch.float32, device="cuda")
+            arr[i] = NpmgQueue(_ptr(q["inputs"]), _ptr(q.get("normal")) if flags else None, _ptr(q.get("u")),
+                               _ptr(q.get("eval_dirs")), _ptr(o.get("dir")), _ptr(o.get("pdf")),
+                               _ptr(o.get("eval_pdf")), _ptr(o.get("lobe")), cap)
+            outs.append(o)
+        self._check(self._lib.npmg_sample_pdf_queues(self._h, arr, _ptr(counts), nq, n_eval, flags, _stream()))
+        return outs
+
+    def autotune(self, n: int):
+        """Times the query path's launch shapes for batches of about n; returns (chunks, blocks_per_cu)."""
+        self._check(self._lib.npmg_autotune(self._h, n, _stream()))
+        return self.tuning()
+
+    def tuning(self):
+        c, b = _u32(), _u32()
+        self._check(self._lib.npmg_tuning(self._h, ctypes.byref(c), ctypes.byref(b)))
+        return c.value, b.value
 
     def train_step_graph(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, loss_out):
         """train_step as one captured HIP graph (replayed while the buffers are the same)."""
