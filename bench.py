@@ -25,6 +25,8 @@ sys.path.insert(0, os.path.join(ROOT, "neural-mixture-guiding_amd"))
 METRIC = "guided queries/s (infer+sample+pdf) & train samples/s at 2M batch, BF16 vs FP32"
 N_QUERIES = 1 << 21
 N_TRAIN = 1 << 18
+FRAME_PATHS = 1920 * 1080  # one wavefront bounce of a 1920x1080 frame
+FRAME_BOUNCES = 8
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -96,8 +98,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="all", choices=["all", "query"],
-                    help="query: time only the fused query kernel (for counter runs)")
+    ap.add_argument("--only", default="all", choices=["all", "query", "frame"],
+                    help="query: time only the fused query kernel (for counter runs); "
+                         "frame: only the anisotropic 8-bounce frame")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy 5 times (HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -159,7 +162,7 @@ def main():
     t = npmg.synth_train(77, N_TRAIN)
     loss = torch.empty(1, device="cuda")
 
-    def measure_train(netp, precision, graph=False):
+    def measure_train(netp, precision, graph=False):  # precision: label for the log
         fn = lib.npmg_train_step_graph if graph else lib.npmg_train_step
 
         def train_step():
@@ -191,7 +194,82 @@ def main():
         log(f"{precision}: held-out loss after {steps} steps {value:.4f}")
         return value
 
+    def pdf_agreement(make_cfg, precision, cosine, train_steps=200, n=1 << 16):
+        """Relative pdf difference of the `precision` path against the FP32 path on
+        identical parameters (an FP32 network trained for `train_steps` steps on
+        the analytic field, copied into the other): median, 99th percentile, max
+        over the 4 evaluation pdfs of n queries."""
+        ref = npmg.GuidingNetwork(make_cfg("fp32"), seed=7272)
+        batch = npmg.synth_train(123, N_TRAIN)
+        for k in range(train_steps):
+            lib.npmg_synth_train(123, k * N_TRAIN, N_TRAIN, P(batch["inputs"]), P(batch["tgt_dir"]),
+                                 P(batch["tgt_rgb"]), P(batch["tgt_pdf"]), stream.cuda_stream)
+            ref.train_step_graph(batch["inputs"], batch["tgt_dir"], batch["tgt_rgb"], batch["tgt_pdf"], loss)
+        other = npmg.GuidingNetwork(make_cfg(precision), seed=7272)
+        other.set_params(ref.get_params(1))
+        ref.set_params(ref.get_params(1))
+        qq = npmg.synth_queries(555, n)
+        nrm = qq["normal"] if cosine else None
+        a = ref.sample_pdf(qq["inputs"], nrm, None, qq["eval_dirs"], cosine_aware=cosine)["eval_pdf"]
+        b = other.sample_pdf(qq["inputs"], nrm, None, qq["eval_dirs"], cosine_aware=cosine)["eval_pdf"]
+        rel = ((b.double() - a.double()).abs() / a.double()).flatten()
+        ref.close()
+        other.close()
+        return {"median": rel.median().item(), "p99": rel.quantile(0.99).item() if rel.numel() <= 1 << 24 else None,
+                "max": rel.max().item(), "queries": n, "trained_steps": train_steps}
+
+    def asg_cfg(precision):
+        c = npmg.benchmark_config("full", precision=precision, width=128, hidden=3)
+        c["mixture"] = {"otype": "ASG", "n_lobes": 16}
+        return c
+
+    def measure_frame(precision):
+        """One wavefront frame of the anisotropic network: 8 bounce queues of
+        floor(2,073,600 * 0.68^b) queries (capacity 2,073,600 each) served by one
+        persistent npmg_sample_pdf_queues launch; then the per-frame train step
+        (2^18 samples) as a graph."""
+        netf = npmg.GuidingNetwork(asg_cfg(precision), seed=7272)
+        tuned = netf.autotune(1 << 21) if precision != "fp32" else netf.tuning()
+        counts = [int(FRAME_PATHS * 0.68 ** b) for b in range(FRAME_BOUNCES)]
+        cap = FRAME_PATHS
+        qin = torch.empty(FRAME_BOUNCES, 9, cap, device="cuda")
+        qu = torch.empty(FRAME_BOUNCES, 3, cap, device="cuda")
+        qe = torch.empty(FRAME_BOUNCES, 4, 3, cap, device="cuda")
+        scratch_n = torch.empty(3, cap, device="cuda")
+        for b in range(FRAME_BOUNCES):
+            lib.npmg_synth_queries(900 + b, 0, cap, P(qin[b]), P(scratch_n), P(qu[b]), P(qe[b]), stream.cuda_stream)
+        odir = torch.empty(FRAME_BOUNCES, 3, cap, device="cuda")
+        opdf = torch.empty(FRAME_BOUNCES, cap, device="cuda")
+        oev = torch.empty(FRAME_BOUNCES, 4, cap, device="cuda")
+        olobe = torch.empty(FRAME_BOUNCES, cap, dtype=torch.int32, device="cuda")
+        arr = (npmg.NpmgQueue * FRAME_BOUNCES)()
+        for b in range(FRAME_BOUNCES):
+            arr[b] = npmg.NpmgQueue(P(qin[b]), None, P(qu[b]), P(qe[b]), P(odir[b]), P(opdf[b]), P(oev[b]),
+                                    P(olobe[b]), cap)
+        cnt = torch.tensor(counts, dtype=torch.int32, device="cuda")
+
+        def frame_step():
+            st = lib.npmg_sample_pdf_queues(netf._h, arr, P(cnt), FRAME_BOUNCES, 4, 0, stream.cuda_stream)
+            if st != 0:
+                raise npmg.NpmgError(st, lib.npmg_last_error(netf._h).decode())
+
+        ms = time_region(frame_step, max(2, args.steps // 2), 2, stream)
+        total = sum(counts)
+        log(f"ASG frame {precision}: {ms:.3f} ms per frame of {total} queries in 8 queues "
+            f"({total / ms / 1e3:.1f} M queries/s), tuning {tuned}")
+        ms_train = measure_train(netf, precision + " ASG", graph=True)
+        netf.close()
+        del qin, qu, qe, odir, opdf, oev, olobe
+        return {"ms_per_frame": ms, "queries_per_frame": total, "queries_per_s": total / (ms * 1e-3),
+                "train_graph_ms": ms_train, "train_samples_per_s": N_TRAIN / (ms_train * 1e-3),
+                "tuning": {"pipeline_chunks": tuned[0], "blocks_per_cu": tuned[1]}}
+
     per_prec, per_prec_train, per_prec_graph, per_kernel = {}, {}, {}, {}
+    if args.only == "frame":
+        frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, args.precision, False, train_steps=100)
+        print(json.dumps(frame), flush=True)
+        return
     for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
         netp, ms = measure_queries(prec)
         per_prec[prec] = ms
@@ -256,6 +334,17 @@ def main():
         "train_samples_per_s_graph_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_graph.items()},
         "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16")},
     }
+    if half:
+        result["pdf_rel_diff_vs_fp32_same_params"] = {
+            p: pdf_agreement(lambda pr: npmg.benchmark_config("full", precision=pr), p, True)
+            for p in ("bf16", "fp16")}
+        frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
+        frame["workload"] = ("one frame of 8 bounce queues of floor(2,073,600 * 0.68^b) queries, one persistent "
+                             "launch; anisotropic mixture 16 lobes x 6, MLP 3x128, hash grid 16x2^19x2, "
+                             "sample + 4 eval pdfs; train step 2^18 samples as one graph")
+        frame[f"{args.precision}_speedup_vs_fp32"] = frame["fp32"]["ms_per_frame"] / frame[args.precision]["ms_per_frame"]
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, args.precision, False, train_steps=100)
+        result["anisotropic_frame"] = frame
     if "fp32" in per_prec and half:
         result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]
         result[f"{args.precision}_train_speedup_vs_fp32"] = per_prec_train["fp32"] / per_prec_train[args.precision]
