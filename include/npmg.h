@@ -156,6 +156,22 @@ npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream);
  * many as fit). */
 npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_per_cu);
 
+/* FP8 inference ("inference_precision": "fp8" with "precision": "bf16"): the
+ * query path's MLP runs on E4M3 weights and activations with FP32
+ * accumulation (v_mfma_f32_32x32x16_fp8_fp8); training stays BF16. Each layer
+ * has a power-of-two weight scale (from max |W|, recomputed whenever the
+ * inference parameters change) and input scale (from the maximum activation
+ * of an FP32 forward over calibration queries, with one binade of headroom;
+ * larger values saturate). The first inference call calibrates on its first
+ * 65536 queries unless npmg_fp8_calibrate ran before; npmg_fp8_calibrate
+ * recalibrates (in [9][n], at most 65536 queries used; synchronous). No
+ * counterpart in the reference, whose inference runs tcnn's FP16 network
+ * (integrator.cpp:645). */
+npmg_status npmg_fp8_calibrate(npmg_ctx* ctx, const float* in, uint32_t n, npmg_stream stream);
+/* The scales in use (host arrays of n_layers = hidden layers + 1 entries):
+ * layer l computes (W_l sw_l)(x_l sx_l) in E4M3. Synchronous. */
+npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_scales, uint32_t n_layers);
+
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 
 /* Encoding only: in [9][n] -> enc [encoded_width][n]. */
@@ -200,7 +216,8 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
 
 /* Evaluates the library's device elementary functions over x (and y):
  * fn 0 exp, 1 log, 2 log1p, 3 expm1, 4 sin, 5 cos, 6 acos, 7 atan2(x, y),
- * 8 sqrt, 9 logistic, 10 x / y. For bitwise host/device comparisons. */
+ * 8 sqrt, 9 logistic, 10 x / y, 11 x rounded to FP8 E4M3 and back (the
+ * conversion of the FP8 inference path). For bitwise host/device comparisons. */
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream);
 
 /* ---- Kernel timing ---------------------------------------------------------

@@ -28,9 +28,10 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 size_t wgrad_partials_needed(const DevCfg& c, uint32_t n, uint32_t* n_splits, uint32_t* q_per_split);
 hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, const void* genc,
                              const QueryIO& io, uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu,
-                             int blocks_per_cu, hipStream_t s);
+                             int blocks_per_cu, const float* fp8_scale, hipStream_t s);
 hipError_t launch_query_mfma_queues(const DevCfg& c, int precision, const void* wfrag, const void* table,
-                                    const QueueSet& qs, int n_cu, int blocks_per_cu, hipStream_t s);
+                                    const QueueSet& qs, int n_cu, int blocks_per_cu, const float* fp8_scale,
+                                    hipStream_t s);
 hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void* table, const QueueSet& qs, int n_cu,
                                      hipStream_t s);
 hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
@@ -197,6 +198,12 @@ struct npmg_ctx {
     uint32_t tune_chunks = 4;
     int tune_bpc = 0;
     uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
+    // FP8 inference: calibrated input maxima per layer and the scales derived from them
+    // (wscale [L], ascale [L], qscale [L + 1]); calibration uses at most kFp8CalibMax queries
+    static constexpr uint32_t kFp8CalibMax = 1u << 16;
+    float* fp8_amax = nullptr;
+    float* fp8_scales = nullptr;
+    bool fp8_calibrated = false;
     // kernel timing: event pairs per tag, recorded while enabled, read by npmg_timing_read
     static constexpr int kTags = 3;
     bool timing = false;
@@ -235,7 +242,7 @@ struct npmg_ctx {
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
                         (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes,
-                        (void*)queue_counter})
+                        (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales})
             if (p) (void)hipFree(p);
     }
 
@@ -304,7 +311,10 @@ struct npmg_ctx {
         if (half()) {
             hip_check(launch_pack_frags(cfg.d, cfg.precision, master, frag_train, s), "pack");
             if (bfrag_train) hip_check(launch_pack_bfrags(cfg.d, cfg.precision, master, bfrag_train, s), "pack");
-            hip_check(launch_pack_frags(cfg.d, cfg.precision, infer_params(), frag_infer, s), "pack");
+            if (cfg.infer_fp8)
+                hip_check(launch_pack_frags_fp8(cfg.d, infer_params(), frag_infer, fp8_amax, fp8_scales, s), "pack");
+            else
+                hip_check(launch_pack_frags(cfg.d, cfg.precision, infer_params(), frag_infer, s), "pack");
             if (tables) {
                 hip_check(launch_to_half(cfg.precision, master + cfg.n_mlp, htab_train, cfg.n_grid, s), "table");
                 hip_check(launch_to_half(cfg.precision, infer_params() + cfg.n_mlp, htab_infer, cfg.n_grid, s),
@@ -318,9 +328,14 @@ struct npmg_ctx {
         if (half()) {
             const void* frag = inference ? frag_infer : frag_train;
             const void* tab = inference ? htab_infer : htab_train;
+            const float* q8 = nullptr;
+            if (inference && cfg.infer_fp8) {
+                if (!fp8_calibrated && n) calibrate_fp8(io.in, n, std::min(n, kFp8CalibMax), s);
+                q8 = fp8_scales + 2 * cfg.d.n_layers;
+            }
             if (!mfma_split_grid(cfg.d)) {
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, tune_bpc, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
                 return;
@@ -337,7 +352,7 @@ struct npmg_ctx {
                               "grid encode");
                 });
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, tune_bpc, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
                 return;
@@ -356,7 +371,7 @@ struct npmg_ctx {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 hip_check(hipStreamWaitEvent(s, ev_chunk[k], 0), "stream wait");
                 timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, tune_bpc, s),
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
             }
@@ -416,7 +431,10 @@ struct npmg_ctx {
                               "grid encode (queues)");
                 }
                 qs.counter = queue_counter + 1;
-                hip_check(launch_query_mfma_queues(cfg.d, cfg.precision, frag_infer, htab_infer, qs, n_cu, tune_bpc, s),
+                need(!cfg.infer_fp8 || fp8_calibrated,
+                     "FP8 inference is not calibrated: call npmg_fp8_calibrate (or one plain query) first");
+                hip_check(launch_query_mfma_queues(cfg.d, cfg.precision, frag_infer, htab_infer, qs, n_cu, tune_bpc,
+                                                   cfg.infer_fp8 ? fp8_scales + 2 * cfg.d.n_layers : nullptr, s),
                           "query (mfma, queues)");
             } else {
                 qs.counter = queue_counter + 1;
@@ -436,6 +454,11 @@ struct npmg_ctx {
         float* buf = nullptr;
         hip_check(hipMalloc((void**)&buf, bytes), "hipMalloc");
         hipEvent_t e0 = nullptr, e1 = nullptr;
+        // restored on every exit: timing is off while tuning, and the FP8 path is
+        // timed without calibrating it on the synthetic queries
+        const bool was_timing = timing, was_calibrated = fp8_calibrated;
+        const uint32_t old_c = tune_chunks;
+        const int old_b = tune_bpc;
         try {
             float *in = buf, *nrm = in + 9 * (size_t)n, *u = nrm + 3 * (size_t)n, *ed = u + 3 * (size_t)n;
             float *od = ed + 12 * (size_t)n, *op = od + 3 * (size_t)n;
@@ -448,8 +471,8 @@ struct npmg_ctx {
             io.out_pdf = op;
             hip_check(hipEventCreate(&e0), "event");
             hip_check(hipEventCreate(&e1), "event");
-            const bool was_timing = timing;
             timing = false;
+            fp8_calibrated = true;
             float best = 1e30f;
             uint32_t best_c = tune_chunks;
             int best_b = tune_bpc;
@@ -474,9 +497,14 @@ struct npmg_ctx {
                 }
             }
             timing = was_timing;
+            fp8_calibrated = was_calibrated;
             tune_chunks = best_c;
             tune_bpc = best_b;
         } catch (...) {
+            timing = was_timing;
+            fp8_calibrated = was_calibrated;
+            tune_chunks = old_c;
+            tune_bpc = old_b;
             (void)hipStreamSynchronize(s);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
@@ -486,6 +514,39 @@ struct npmg_ctx {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         hip_check(hipFree(buf), "hipFree");
+    }
+
+    // Activation maxima of the inference parameters' FP32 forward over m queries of
+    // `in` (row stride n), then the FP8 fragments and scales. Synchronous.
+    void calibrate_fp8(const float* in, uint32_t n, uint32_t m, hipStream_t s) {
+        if (!cfg.infer_fp8) throw std::invalid_argument("inference_precision is not fp8");
+        need(m > 0 && m <= n && in, "calibration needs queries");
+        const DevCfg& d = cfg.d;
+        float* buf = nullptr;
+        const size_t n_tin = (size_t)d.n_in * m, n_enc = (size_t)d.enc_padded * m;
+        const size_t n_act = (size_t)(d.n_layers - 1) * d.width * m;
+        hip_check(hipMalloc((void**)&buf, (n_tin + n_enc + n_act) * sizeof(float)), "hipMalloc");
+        try {
+            float* tin = buf;
+            TrainWS t{};
+            t.enc = buf + n_tin;
+            t.acts = t.enc + n_enc;
+            hip_check(hipMemcpy2DAsync(tin, (size_t)m * sizeof(float), in, (size_t)n * sizeof(float),
+                                       (size_t)m * sizeof(float), (size_t)d.n_in, hipMemcpyDeviceToDevice, s),
+                      "copy calibration queries");
+            hip_check(launch_train_forward_fp32(d, wt_infer, infer_params() + cfg.n_mlp, tin, nullptr, nullptr,
+                                                nullptr, LossScale{}, t, m, n_cu, s),
+                      "calibration forward");
+            hip_check(launch_fp8_amax(d, t.enc, t.acts, m, fp8_amax, s), "calibration maxima");
+            hip_check(launch_pack_frags_fp8(d, infer_params(), frag_infer, fp8_amax, fp8_scales, s), "pack");
+            hip_check(hipStreamSynchronize(s), "synchronize");
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(buf);
+            throw;
+        }
+        hip_check(hipFree(buf), "hipFree");
+        fp8_calibrated = true;
     }
 
     void free_ws() {
@@ -738,7 +799,8 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         if (ctx->half()) {
             need(c.n_mlp % 512 == 0, "half precision needs layer sizes that are multiples of 32 x 16");
             ctx->frag_train = dalloc<uint16_t>(c.n_mlp);
-            ctx->frag_infer = dalloc<uint16_t>(c.n_mlp);
+            if (c.infer_fp8) ctx->frag_infer = dalloc<uint8_t>(c.n_mlp);
+            else ctx->frag_infer = dalloc<uint16_t>(c.n_mlp);
             ctx->htab_train = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
             ctx->htab_infer = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
             if (mfma_train_supported(c.d)) ctx->bfrag_train = dalloc<uint16_t>(bfrag_elements(c.d));
@@ -746,6 +808,14 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         ctx->sstate = dalloc<ScaleState>(1);
         ctx->mean_partials = dalloc<float>(256);
         ctx->queue_counter = dalloc<uint32_t>(2);
+        if (c.infer_fp8) {
+            // until calibrated, unit maxima (valid scales; the first query calibrates)
+            ctx->fp8_amax = dalloc<float>(kMaxLayers);
+            ctx->fp8_scales = dalloc<float>(3 * kMaxLayers + 1);
+            const std::vector<float> ones(kMaxLayers, 1.0f);
+            hip_check(hipMemcpy(ctx->fp8_amax, ones.data(), kMaxLayers * sizeof(float), hipMemcpyHostToDevice),
+                      "upload");
+        }
         const ScaleState st0{c.loss_scale, 0u, 0u, 0u};
         hip_check(hipMemcpy(ctx->sstate, &st0, sizeof(st0), hipMemcpyHostToDevice), "upload");
         // decay^t for the EMA debiasing (tcnn ema.h:110-111 evaluates std::pow in
@@ -902,6 +972,26 @@ npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream) {
     return guarded(ctx, [&] {
         ctx->check_fused();
         ctx->autotune(n, S(stream));
+    });
+}
+
+npmg_status npmg_fp8_calibrate(npmg_ctx* ctx, const float* in, uint32_t n, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n > 0 && in, "calibration needs queries");
+        ctx->calibrate_fp8(in, n, std::min(n, npmg_ctx::kFp8CalibMax), S(stream));
+    });
+}
+
+npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_scales, uint32_t n_layers) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        if (!ctx->cfg.infer_fp8) throw std::invalid_argument("inference_precision is not fp8");
+        const int L = ctx->cfg.d.n_layers;
+        need(n_layers == (uint32_t)L && weight_scales && input_scales, "need n_layers entries per array");
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        hip_check(hipMemcpy(weight_scales, ctx->fp8_scales, L * sizeof(float), hipMemcpyDeviceToHost), "download");
+        hip_check(hipMemcpy(input_scales, ctx->fp8_scales + L, L * sizeof(float), hipMemcpyDeviceToHost), "download");
     });
 }
 
@@ -1114,7 +1204,7 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
 
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream) {
     if (n && !(x && out)) return NPMG_EINVAL;
-    if (fn < 0 || fn > 10) return NPMG_EINVAL;
+    if (fn < 0 || fn > 11) return NPMG_EINVAL;
     return launch_debug_math(fn, x, y, n, out, S(stream)) == hipSuccess ? NPMG_OK : NPMG_EHIP;
 }
 

@@ -45,6 +45,9 @@ struct DevCfg {
 struct Config {
     DevCfg d{};
     Precision precision = PREC_FP32;
+    // inference (query path with the inference parameters) in FP8 E4M3 with
+    // per-layer scales; training stays in `precision` (BF16)
+    bool infer_fp8 = false;
     // Adam (tiny-cuda-nn optimizers/adam.h) and EMA (optimizers/ema.h)
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, l2_reg = 1e-6f;
     bool ema = true;

@@ -1,0 +1,39 @@
+// k_query_mfma_inst.hip -- instantiations of the fused MFMA query kernel,
+// built once per (operand type, lobe family) with
+//   -DNPMG_INST_{bf16,f16,fp8} -DNPMG_INST_{vmf,asg}
+// so that the six objects compile in parallel.
+#include "query_mfma.h"
+
+namespace npmg {
+
+#if defined(NPMG_INST_bf16)
+#define NPMG_TYPES __bf16, __bf16
+#elif defined(NPMG_INST_f16)
+#define NPMG_TYPES _Float16, _Float16
+#elif defined(NPMG_INST_fp8)
+#define NPMG_TYPES __bf16, mf::Fp8  // BF16 grid table and features, E4M3 MLP operands
+#else
+#error "operand type: define NPMG_INST_bf16, NPMG_INST_f16 or NPMG_INST_fp8"
+#endif
+
+#define NPMG_INST(WW, KK, PP)                                                                                      \
+    template hipError_t launch_mfma_t<NPMG_TYPES, WW, KK, PP>(                                                     \
+        const DevCfg&, const void*, const void*, const void*, const QueryIO&, uint32_t, uint32_t, uint32_t, int, \
+        int, const float*, hipStream_t);                                                                           \
+    template hipError_t launch_mfma_queues_t<NPMG_TYPES, WW, KK, PP>(const DevCfg&, const void*, const void*,      \
+                                                                     const QueueSet&, int, int, const float*,      \
+                                                                     hipStream_t);
+
+#if defined(NPMG_INST_vmf)
+NPMG_INST(64, 8, 4)
+NPMG_INST(64, 16, 4)
+NPMG_INST(128, 8, 4)
+NPMG_INST(128, 16, 4)
+#elif defined(NPMG_INST_asg)
+NPMG_INST(64, 16, 6)
+NPMG_INST(128, 16, 6)
+#else
+#error "lobe family: define NPMG_INST_vmf or NPMG_INST_asg"
+#endif
+
+}  // namespace npmg

@@ -113,6 +113,12 @@ hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const Tra
 // grid_scatter_bytes; its first grid_scatter_clear_bytes must be zero before
 // the first call (the scatter leaves them zero again).
 size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic);
+// FP8 inference: fragments with per-layer weight scales and the fused kernel's
+// multipliers (fp8 = wscale [L], ascale [L], qscale [L + 1]); calibration maxima.
+hipError_t launch_pack_frags_fp8(const DevCfg& c, const float* w, void* frag, const float* amax, float* fp8,
+                                 hipStream_t s);
+hipError_t launch_fp8_amax(const DevCfg& c, const float* enc, const float* acts, uint32_t m, float* amax,
+                           hipStream_t s);
 // Zero fill by a kernel (safe inside captured graphs, unlike a memset node).
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic);

@@ -28,6 +28,13 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
+// FP8 E4M3 operand byte (inference only). v_mfma_f32_32x32x16_fp8_fp8 has the
+// fragment layout of the 16-bit types above with 8 one-byte values per lane.
+struct Fp8 {
+    uint8_t bits;
+};
+constexpr float kFp8Max = 448.0f;  // largest finite E4M3 value
+
 // Hidden neuron read by logical input k = 16 kb + 8 h + e of a chained layer.
 __host__ __device__ __forceinline__ int chained_column(int kb, int h, int e) {
     return 32 * (kb >> 1) + 8 * (2 * (kb & 1) + (e >> 2)) + 4 * h + (e & 3);
@@ -47,21 +54,47 @@ template <> struct Frag<_Float16> {
     }
 };
 
-// 8 floats -> one B/A fragment (round to nearest even).
+template <> struct Frag<Fp8> {
+    using vec = long;  // 8 E4M3 values, element e in byte e
+    __device__ static __forceinline__ f32x16 mma(vec a, vec b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, c, 0, 0, 0);
+    }
+};
+
+// 8 floats -> one B/A fragment (round to nearest even; FP8 inputs must lie
+// within +-kFp8Max, the callers clamp).
 template <class T>
 __device__ __forceinline__ typename Frag<T>::vec pack8(const float* v) {
-    typename Frag<T>::vec r;
+    if constexpr (sizeof(typename Frag<T>::vec) == 8) {
+        int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+        int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
+        hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+        return (long)(((unsigned long)(uint32_t)hi << 32) | (unsigned long)(uint32_t)lo);
+    } else {
+        typename Frag<T>::vec r;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = (T)v[e];
-    return r;
+        for (int e = 0; e < 8; ++e) r[e] = (T)v[e];
+        return r;
+    }
 }
 
-// Fragment of the weights: 16 bytes per lane, fragments stored lane-major so
-// a wave reads 1 KB contiguously (conflict-free ds_read_b128).
+// One float -> E4M3 byte (round to nearest even; |x| <= kFp8Max).
+__device__ __forceinline__ uint8_t to_fp8(float x) {
+    return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+}
+
+// Fragment of the weights, fragments stored lane-major so a wave reads one
+// contiguous block (16 bytes per lane, or 8 for FP8): conflict-free LDS reads.
 template <class T>
 __device__ __forceinline__ typename Frag<T>::vec load_frag(const T* base, int frag, int lane) {
-    const uint4 raw = reinterpret_cast<const uint4*>(base)[frag * 64 + lane];
-    return __builtin_bit_cast(typename Frag<T>::vec, raw);
+    if constexpr (sizeof(typename Frag<T>::vec) == 8) {
+        const uint2 raw = reinterpret_cast<const uint2*>(base)[frag * 64 + lane];
+        return __builtin_bit_cast(long, raw);
+    } else {
+        const uint4 raw = reinterpret_cast<const uint4*>(base)[frag * 64 + lane];
+        return __builtin_bit_cast(typename Frag<T>::vec, raw);
+    }
 }
 
 }  // namespace mf

@@ -1,0 +1,285 @@
+// query_mfma.h -- the fused MFMA query kernel (included by
+// k_query_mfma_inst.hip only, which instantiates it per operand type and lobe
+// family so the instantiations compile in parallel).
+//
+// One wave handles 64 queries. Lane l = 32 h + j encodes, for queries j and
+// 32 + j of the tile, the feature chunks 2 kb + h (8 features each) of every
+// K-block kb of the first layer, and hands them to v_mfma_f32_32x32x16 as B
+// fragments straight from registers. Weights live in LDS in
+// fragment order (1 KB per 32x16 fragment, conflict-free reads). Hidden
+// layers chain register to register (see mfma.h). The output tile is split
+// between lanes l and l ^ 32 and exchanged with one shuffle, after which
+// lane l owns every raw output of query l and runs the FP32 mixture code
+// (normaliser, softmax, log-sum-exp and sampling stay FP32).
+#pragma once
+
+#include "asg.h"
+#include "device.h"
+#include "kernels.h"
+#include "mfma.h"
+
+namespace npmg {
+namespace {
+
+using namespace dev;
+
+
+// Queries [q_begin, q_end) of SoA arrays with row stride n.
+// One 64-query tile [q0, min(q0 + 64, q_end)) of a query stream with row stride n:
+// encode, MLP on MFMA (weights in LDS at sw), mixture, sample and pdfs.
+//
+// O is the MFMA operand type: T itself, or mf::Fp8 for FP8 inference, where
+// qscale holds the per-layer power-of-two multipliers: [0] input features ->
+// E4M3, [l] accumulators of layer l - 1 -> E4M3 inputs of layer l, [n_layers]
+// output accumulators -> raw outputs.
+template <class T, class O, int W, int K, int P>
+__device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, const T* __restrict__ table,
+                                                const T* __restrict__ genc, const QueryIO& io, uint32_t n,
+                                                uint32_t q0, uint32_t q_end, int lane,
+                                                const float* __restrict__ qscale) {
+    using V = typename mf::Frag<O>::vec;
+    constexpr bool F8 = sizeof(V) == 8;
+    constexpr int NOUT = (P * K + 15) / 16 * 16;
+    static_assert(NOUT % 32 == 0, "whole 32-row output tiles");
+    constexpr int TO = NOUT / 32;  // output tiles
+    constexpr int TH = W / 32;     // hidden tiles
+    constexpr int KBH = W / 16;    // K-blocks of a hidden layer's input
+    const int h = lane >> 5, j = lane & 31;
+    const int KB0 = c.enc_padded / 16;
+    const uint32_t qa = min(q0 + (uint32_t)j, q_end - 1u), qb = min(q0 + 32u + (uint32_t)j, q_end - 1u);
+    float ina[16], inb[16];
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+        ina[d] = d < c.n_in ? io.in[(size_t)d * n + qa] : 0.0f;
+        inb[d] = d < c.n_in ? io.in[(size_t)d * n + qb] : 0.0f;
+    }
+    mf::f32x16 acc[TH][2];
+#pragma unroll
+    for (int t = 0; t < TH; ++t) acc[t][0] = acc[t][1] = mf::f32x16{};
+    // Layer 0: encode chunk by chunk straight into B fragments.
+    for (int kb = 0; kb < KB0; ++kb) {
+        float va[8], vb[8];
+        encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
+        encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
+        if constexpr (F8) {
+            const float s0 = qscale[0];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                va[e] = __builtin_amdgcn_fmed3f(va[e] * s0, -mf::kFp8Max, mf::kFp8Max);
+                vb[e] = __builtin_amdgcn_fmed3f(vb[e] * s0, -mf::kFp8Max, mf::kFp8Max);
+            }
+        }
+        const V ba = mf::pack8<O>(va), bb = mf::pack8<O>(vb);
+#pragma unroll
+        for (int t = 0; t < TH; ++t) {
+            const V a = mf::load_frag<O>(sw, t * KB0 + kb, lane);
+            acc[t][0] = mf::Frag<O>::mma(a, ba, acc[t][0]);
+            acc[t][1] = mf::Frag<O>::mma(a, bb, acc[t][1]);
+        }
+    }
+    int fb = TH * KB0;
+    // Hidden layers: ReLU, round, and feed the accumulators back as B fragments.
+    for (int l = 1; l < c.n_layers - 1; ++l) {
+        const float m = F8 ? qscale[l] : 1.0f;
+        V bx[2][KBH];
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                float v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float a = acc[kb >> 1][cc][8 * (kb & 1) + e];
+                    if constexpr (F8) v[e] = __builtin_amdgcn_fmed3f(a * m, 0.0f, mf::kFp8Max);
+                    else v[e] = a > 0.0f ? a : 0.0f;
+                }
+                bx[cc][kb] = mf::pack8<O>(v);
+            }
+#pragma unroll
+        for (int t = 0; t < TH; ++t) {
+            acc[t][0] = acc[t][1] = mf::f32x16{};
+#pragma unroll
+            for (int kb = 0; kb < KBH; ++kb) {
+                const V a = mf::load_frag<O>(sw, fb + t * KBH + kb, lane);
+                acc[t][0] = mf::Frag<O>::mma(a, bx[0][kb], acc[t][0]);
+                acc[t][1] = mf::Frag<O>::mma(a, bx[1][kb], acc[t][1]);
+            }
+        }
+        fb += TH * KBH;
+    }
+    // Output layer.
+    const float m_last = F8 ? qscale[c.n_layers - 1] : 1.0f;
+    V bx[2][KBH];
+#pragma unroll
+    for (int kb = 0; kb < KBH; ++kb)
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float a = acc[kb >> 1][cc][8 * (kb & 1) + e];
+                if constexpr (F8) v[e] = __builtin_amdgcn_fmed3f(a * m_last, 0.0f, mf::kFp8Max);
+                else v[e] = a > 0.0f ? a : 0.0f;
+            }
+            bx[cc][kb] = mf::pack8<O>(v);
+        }
+    const float m_out = F8 ? qscale[c.n_layers] : 1.0f;
+    float out[NOUT];
+#pragma unroll
+    for (int t = 0; t < TO; ++t) {
+        mf::f32x16 o0 = {}, o1 = {};
+#pragma unroll
+        for (int kb = 0; kb < KBH; ++kb) {
+            const V a = mf::load_frag<O>(sw, fb + t * KBH + kb, lane);
+            o0 = mf::Frag<O>::mma(a, bx[0][kb], o0);
+            o1 = mf::Frag<O>::mma(a, bx[1][kb], o1);
+        }
+        if constexpr (F8) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                o0[r] *= m_out;
+                o1[r] *= m_out;
+            }
+        }
+        // Lane (h, j) keeps the tile of its own query (c = h) and swaps the other with lane l ^ 32.
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float mine = h ? o1[r] : o0[r];
+            const float recv = __shfl_xor(h ? o0[r] : o1[r], 32);
+            const int a = r >> 2, b = r & 3;
+            out[32 * t + 8 * a + b] = h ? recv : mine;
+            out[32 * t + 8 * a + 4 + b] = h ? mine : recv;
+        }
+    }
+    const uint32_t q = q0 + (uint32_t)lane;
+    if (q < q_end) {
+        if (io.raw_out) {
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
+        }
+        if constexpr (P == 6) {
+            if (io.u || io.n_eval > 0) asg_query<K>([&](int jj) { return out[jj]; }, io, q, n);
+        } else if (io.u || io.n_eval > 0) {
+            Mixture<K> m;
+            build_mixture<K>(c, [&](int jj) { return out[jj]; }, m);
+            float th[K], ph[K];
+            if (io.normal) {
+                const V3 nrm = mk(io.normal[q], io.normal[(size_t)n + q], io.normal[2 * (size_t)n + q]);
+                apply_cosine<K>(c, nrm, m, false, th, ph);
+            }
+            if (io.u) {
+                uint32_t lobe;
+                const V3 d = mixture_sample<K>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
+                io.out_dir[q] = d.x;
+                io.out_dir[(size_t)n + q] = d.y;
+                io.out_dir[2 * (size_t)n + q] = d.z;
+                io.out_pdf[q] = mixture_pdf<K>(c, m, d);
+                if (io.out_lobe) io.out_lobe[q] = lobe;
+            }
+            for (int e = 0; e < io.n_eval; ++e) {
+                const float* ed = io.eval_dirs + (size_t)e * 3 * n;
+                io.out_eval_pdf[(size_t)e * n + q] =
+                    mixture_pdf<K>(c, m, mk(ed[q], ed[(size_t)n + q], ed[2 * (size_t)n + q]));
+            }
+        }
+    }
+}
+
+// Queries [q_begin, q_end) of SoA arrays with row stride n.
+template <class T, class O, int W, int K, int P>
+__global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const O* __restrict__ wfrag,
+                                                    const T* __restrict__ table, const T* __restrict__ genc,
+                                                    const QueryIO io, uint32_t n, uint32_t n_frag, uint32_t q_begin,
+                                                    uint32_t q_end, const float* __restrict__ qscale) {
+    extern __shared__ uint4 smem_frag[];
+    constexpr uint32_t u4_per_frag = 64u * sizeof(typename mf::Frag<O>::vec) / 16u;
+    for (uint32_t i = threadIdx.x; i < n_frag * u4_per_frag; i += 256u)
+        smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
+    __syncthreads();
+    const O* sw = reinterpret_cast<const O*>(smem_frag);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t n_tiles = (q_end - q_begin + 63u) / 64u;
+    for (uint32_t tile = blockIdx.x * 4u + (uint32_t)wave; tile < n_tiles; tile += gridDim.x * 4u)
+        query_tile_mfma<T, O, W, K, P>(c, sw, table, genc, io, n, q_begin + tile * 64u, q_end, lane, qscale);
+}
+
+// Several queues in one persistent launch: each wave pulls 64-query tiles
+// from a device counter over the concatenated tiles of the queues (counts on
+// the device) until all are taken.
+template <class T, class O, int W, int K, int P>
+__global__ void __launch_bounds__(256) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
+                                                           const T* __restrict__ table, const QueueSet qs,
+                                                           uint32_t n_frag, const float* __restrict__ qscale) {
+    extern __shared__ uint4 smem_frag[];
+    constexpr uint32_t u4_per_frag = 64u * sizeof(typename mf::Frag<O>::vec) / 16u;
+    for (uint32_t i = threadIdx.x; i < n_frag * u4_per_frag; i += 256u)
+        smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
+    __syncthreads();
+    const O* sw = reinterpret_cast<const O*>(smem_frag);
+    const int lane = threadIdx.x & 63;
+    uint32_t first[kMaxQueues + 1];
+    queue_tile_prefix(qs, 64u, first);
+    const uint32_t total = first[kMaxQueues];
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(qs.counter, 1u);
+        w = __shfl(w, 0);  // wave-uniform work item; every wave leaves once the items run out
+        if (w >= total) break;
+        int qi = 0;
+        uint32_t base = 0;
+#pragma unroll
+        for (int i = 1; i < kMaxQueues; ++i)
+            if (w >= first[i] && first[i] < first[i + 1]) {
+                qi = i;
+                base = first[i];
+            }
+        query_tile_mfma<T, O, W, K, P>(c, sw, table, static_cast<const T*>(qs.genc[qi]), qs.io[qi], qs.stride[qi],
+                                       (w - base) * 64u, min(qs.counts[qi], qs.stride[qi]), lane, qscale);
+    }
+}
+
+}  // namespace
+
+template <class T, class O, int W, int K, int P>
+hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const QueryIO& io,
+                         uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, int bpc, const float* qscale,
+                         hipStream_t s) {
+    const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
+    const uint32_t n_frag = n_mlp / 512u;
+    const size_t lds = (size_t)n_frag * 64u * sizeof(typename mf::Frag<O>::vec);
+    auto kern = k_query_mfma<T, O, W, K, P>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    if (bpc > 0 && bpc < per_cu) per_cu = bpc;
+    const uint32_t tiles = (q_end - q_begin + 63u) / 64u;
+    const uint32_t want = (tiles + 3u) / 4u;
+    const uint32_t grid = want < (uint32_t)(n_cu * per_cu) ? want : (uint32_t)(n_cu * per_cu);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, c, static_cast<const O*>(wfrag),
+                       static_cast<const T*>(table), static_cast<const T*>(genc), io, n, n_frag, q_begin, q_end, qscale);
+    return hipGetLastError();
+}
+
+template <class T, class O, int W, int K, int P>
+hipError_t launch_mfma_queues_t(const DevCfg& c, const void* wfrag, const void* table, const QueueSet& qs, int n_cu,
+                                int blocks_per_cu, const float* qscale, hipStream_t s) {
+    const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
+    const uint32_t n_frag = n_mlp / 512u;
+    const size_t lds = (size_t)n_frag * 64u * sizeof(typename mf::Frag<O>::vec);
+    auto kern = k_query_mfma_queues<T, O, W, K, P>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    if (blocks_per_cu > 0 && blocks_per_cu < per_cu) per_cu = blocks_per_cu;
+    hipLaunchKernelGGL(kern, dim3((uint32_t)(n_cu * per_cu)), dim3(256), lds, s, c, static_cast<const O*>(wfrag),
+                       static_cast<const T*>(table), qs, n_frag, qscale);
+    return hipGetLastError();
+}
+
+}  // namespace npmg

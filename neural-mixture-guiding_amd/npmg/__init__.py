@@ -70,6 +70,8 @@ SIGNATURES = {
     "npmg_validate_loss_fd": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, ctypes.c_double, _vp, _vp]),
     "npmg_sample_pdf_queues": (_int, [_vp, ctypes.POINTER(NpmgQueue), _vp, _u32, _u32, _u32, _vp]),
     "npmg_autotune": (_int, [_vp, _u32, _vp]),
+    "npmg_fp8_calibrate": (_int, [_vp, _vp, _u32, _vp]),
+    "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 }
 
@@ -401,6 +403,17 @@ class GuidingNetwork:
         """Times the query path's launch shapes for batches of about n; returns (chunks, blocks_per_cu)."""
         self._check(self._lib.npmg_autotune(self._h, n, _stream()))
         return self.tuning()
+
+    def fp8_calibrate(self, inputs):
+        self._check(self._lib.npmg_fp8_calibrate(self._h, _ptr(inputs), inputs.shape[1], _stream()))
+
+    def fp8_scales(self):
+        """(weight scales, input scales) per layer of the FP8 inference path."""
+        L = self.config["network"]["n_hidden_layers"] + 1
+        w = (_f32 * L)()
+        a = (_f32 * L)()
+        self._check(self._lib.npmg_fp8_scales(self._h, w, a, L))
+        return list(w), list(a)
 
     def tuning(self):
         c, b = _u32(), _u32()
