@@ -133,8 +133,16 @@ def main():
     lib = npmg.lib()
     P = npmg._ptr
 
+    def full_cfg(precision):
+        """The full configuration; "fp8" is BF16 training with FP8 E4M3 inference."""
+        if precision == "fp8":
+            c = npmg.benchmark_config("full", precision="bf16")
+            c["inference_precision"] = "fp8"
+            return c
+        return npmg.benchmark_config("full", precision=precision)
+
     def measure_queries(precision):
-        net = npmg.GuidingNetwork(npmg.benchmark_config("full", precision=precision), seed=7272)
+        net = npmg.GuidingNetwork(full_cfg(precision), seed=7272)
 
         def query_step():
             st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
@@ -143,7 +151,7 @@ def main():
             if st != 0:
                 raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
 
-        ms = time_region(query_step, args.steps, args.warmup, stream)
+        ms = time_region(query_step, args.steps, max(1, args.warmup), stream)  # FP8: the first call calibrates
         log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
         # per-kernel launch durations: HIP events recorded by the library around each
         # launch, on the stream that launch runs on (a separate pass, not the timed one)
@@ -270,10 +278,13 @@ def main():
         frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, args.precision, False, train_steps=100)
         print(json.dumps(frame), flush=True)
         return
-    for prec in ([args.precision] if args.precision == "fp32" else ["fp32", args.precision]):
+    precs = [args.precision] if args.precision == "fp32" else ["fp32", args.precision]
+    if args.precision != "fp32" and args.only != "query":
+        precs.append("fp8")  # inference only
+    for prec in precs:
         netp, ms = measure_queries(prec)
         per_prec[prec] = ms
-        if args.only != "query":
+        if args.only != "query" and prec != "fp8":
             per_prec_train[prec] = measure_train(netp, prec)
             per_prec_graph[prec] = measure_train(netp, prec, graph=True)
         if prec != args.precision:
@@ -336,8 +347,8 @@ def main():
     }
     if half:
         result["pdf_rel_diff_vs_fp32_same_params"] = {
-            p: pdf_agreement(lambda pr: npmg.benchmark_config("full", precision=pr), p, True)
-            for p in ("bf16", "fp16")}
+            p: pdf_agreement(full_cfg, p, True) for p in ("bf16", "fp16", "fp8")}
+        result["fp8_inference_speedup_vs_fp32"] = per_prec["fp32"] / per_prec["fp8"]
         frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
         frame["workload"] = ("one frame of 8 bounce queues of floor(2,073,600 * 0.68^b) queries, one persistent "
                              "launch; anisotropic mixture 16 lobes x 6, MLP 3x128, hash grid 16x2^19x2, "
