@@ -172,6 +172,57 @@ npmg_status npmg_fp8_calibrate(npmg_ctx* ctx, const float* in, uint32_t n, npmg_
  * layer l computes (W_l sw_l)(x_l sx_l) in E4M3. Synchronous. */
 npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_scales, uint32_t n_layers);
 
+/* ---- Wavefront packers (either side of the path) ------------------------ */
+
+/* generate_inference_data (train.cu:174-191): shading points of a queue ->
+ * query records. pos [3][n] world positions; wo [3][n] outgoing directions
+ * and normal [3][n] shading normals (any length, normalised here);
+ * roughness [n]; scene_box: host float[6] = min xyz, max xyz of the scene.
+ * in [9][n]: position normalised to the box inflated by 0.5 % of its
+ * diagonal (normalizeSpatialCoord, train.h:173-177), (w_o + 1) / 2,
+ * normal (theta / pi, phi / 2 pi), 1 - e^-roughness. No context needed. */
+npmg_status npmg_pack_queries(const float* pos, const float* wo, const float* normal, const float* roughness,
+                              const float* scene_box, uint32_t n, float* in, npmg_stream stream);
+
+/* The guided pixel state of a frame (GuidedPixelStateBuffer,
+ * src/render/guided/workqueue.h:15-69, RadianceRecordItem guideditem.h:23-39),
+ * SoA per depth: field[d][c][p] at ((d * C) + c) * n_pixels + p. Angles are
+ * (theta, phi) as cartesianToSpherical writes them. Device pointers. */
+typedef struct npmg_path_records {
+    const float* L;          /* [D][3][P] radiance gathered after the vertex (recordRadiance) */
+    const float* thp;        /* [D][3][P] throughput at the vertex */
+    const float* pos;        /* [D][3][P] world position (ray origin) */
+    const float* dir;        /* [D][2][P] scattered direction */
+    const float* wi_pdf;     /* [D][P] effective pdf of the scattered direction */
+    const float* mis_weight; /* [D][P] */
+    const uint8_t* delta;    /* [D][P] nonzero: sampled from a delta lobe */
+    const float* bsdf_val;   /* [D][3][P] */
+    const float* wo;         /* [D][2][P] outgoing direction */
+    const float* normal;     /* [D][2][P] shading normal */
+    const float* roughness;  /* [D][P] */
+    const uint32_t* depth;   /* [P] recorded vertices per pixel (curDepth) */
+    uint32_t max_depth;      /* D (MAX_TRAIN_DEPTH, parameters.h:8) */
+    uint32_t n_pixels;       /* P */
+} npmg_path_records;
+
+/* Training pixels of a frame (integrator.cpp:666-670): thread t covers pixel
+ * (t % res[0], t / res[0]) * stride + offset, skipped outside resolution. */
+typedef struct npmg_train_tile {
+    uint32_t offset[2], stride[2], res[2], resolution[2];
+} npmg_train_tile;
+
+/* generate_training_data (train.cu:14-67): recorded path vertices -> training
+ * records in (training pixel, depth) order (the reference's atomic push,
+ * train.h:117-124, has no fixed order). Skips delta vertices and records with
+ * a NaN position/direction/target or a zero or NaN pdf; target radiance
+ * (L / thp where thp > 1e-5, else 0) * misWeight * bsdfVal. Outputs with row
+ * stride capacity: in [9], tgt_dir [2] (theta, phi), tgt_rgb [3], tgt_pdf;
+ * *count (device) = records kept; those past capacity are dropped, so train
+ * on min(count, capacity). records, tile, scene_box: host pointers. */
+npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, const npmg_train_tile* tile,
+                               const float* scene_box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
+                               float* tgt_pdf, uint32_t* count, npmg_stream stream);
+
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 
 /* Encoding only: in [9][n] -> enc [encoded_width][n]. */

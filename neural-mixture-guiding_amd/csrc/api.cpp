@@ -198,6 +198,8 @@ struct npmg_ctx {
     uint32_t tune_chunks = 4;
     int tune_bpc = 0;
     uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
+    uint32_t* pack_scratch = nullptr;   // block totals of the training packer
+    size_t pack_cap = 0;
     // FP8 inference: calibrated input maxima per layer and the scales derived from them
     // (wscale [L], ascale [L], qscale [L + 1]); calibration uses at most kFp8CalibMax queries
     static constexpr uint32_t kFp8CalibMax = 1u << 16;
@@ -242,7 +244,7 @@ struct npmg_ctx {
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
                         (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes,
-                        (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales})
+                        (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales, (void*)pack_scratch})
             if (p) (void)hipFree(p);
     }
 
@@ -972,6 +974,62 @@ npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream) {
     return guarded(ctx, [&] {
         ctx->check_fused();
         ctx->autotune(n, S(stream));
+    });
+}
+
+npmg_status npmg_pack_queries(const float* pos, const float* wo, const float* normal, const float* roughness,
+                              const float* scene_box, uint32_t n, float* in, npmg_stream stream) {
+    if (!scene_box || (n && !(pos && wo && normal && roughness && in))) return NPMG_EINVAL;
+    SceneBox box{};
+    for (int a = 0; a < 3; ++a) {
+        box.lo[a] = scene_box[a];
+        box.hi[a] = scene_box[3 + a];
+    }
+    return launch_pack_queries(pos, wo, normal, roughness, box, n, in, S(stream)) == hipSuccess ? NPMG_OK : NPMG_EHIP;
+}
+
+npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, const npmg_train_tile* tile,
+                               const float* scene_box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
+                               float* tgt_pdf, uint32_t* count, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(records && tile && scene_box && count, "null argument");
+        const npmg_path_records& r = *records;
+        need(capacity == 0 || (in && tgt_dir && tgt_rgb && tgt_pdf), "null output");
+        need(tile->res[0] > 0 || tile->res[1] == 0, "training resolution");
+        need(tile->stride[0] > 0 && tile->stride[1] > 0, "stride must be positive");
+        need((uint64_t)tile->resolution[0] * tile->resolution[1] <= r.n_pixels, "resolution exceeds n_pixels");
+        need(r.max_depth <= 64, "max_depth above 64");
+        const uint64_t n_threads = (uint64_t)tile->res[0] * tile->res[1];
+        need(n_threads * r.max_depth < (1ull << 32), "too many records");
+        need(n_threads == 0 || (r.L && r.thp && r.pos && r.dir && r.wi_pdf && r.mis_weight && r.delta &&
+                                r.bsdf_val && r.wo && r.normal && r.roughness && r.depth),
+             "null record field");
+        PathRecords pr{r.L, r.thp, r.pos, r.dir, r.wi_pdf, r.mis_weight, r.delta, r.bsdf_val, r.wo, r.normal,
+                       r.roughness, r.depth, r.max_depth, r.n_pixels};
+        TrainTile tt{};
+        for (int a = 0; a < 2; ++a) {
+            tt.offset[a] = tile->offset[a];
+            tt.stride[a] = tile->stride[a];
+            tt.res[a] = tile->res[a];
+            tt.full[a] = tile->resolution[a];
+        }
+        SceneBox box{};
+        for (int a = 0; a < 3; ++a) {
+            box.lo[a] = scene_box[a];
+            box.hi[a] = scene_box[3 + a];
+        }
+        const size_t need_words = pack_training_scratch(tt);
+        if (need_words > ctx->pack_cap) {
+            if (ctx->pack_scratch) hip_check(hipFree(ctx->pack_scratch), "hipFree");
+            ctx->pack_scratch = nullptr;
+            ctx->pack_cap = 0;
+            ctx->pack_scratch = dalloc<uint32_t>(need_words);
+            ctx->pack_cap = need_words;
+        }
+        hip_check(launch_pack_training(pr, tt, box, capacity, in, tgt_dir, tgt_rgb, tgt_pdf, count,
+                                       ctx->pack_scratch, S(stream)),
+                  "pack training");
     });
 }
 

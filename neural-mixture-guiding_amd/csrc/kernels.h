@@ -113,6 +113,39 @@ hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const Tra
 // grid_scatter_bytes; its first grid_scatter_clear_bytes must be zero before
 // the first call (the scatter leaves them zero again).
 size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic);
+// Wavefront packers (train.cu:14-67, 174-191). Scene box as min xyz, max xyz.
+struct SceneBox {
+    float lo[3], hi[3];
+};
+// The guided pixel state of a frame, SoA per depth (workqueue.h:26-69,
+// guideditem.h:23-44): field[d][c][p] at ((d * C) + c) * n_pixels + p.
+struct PathRecords {
+    const float* L;          // [D][3][P] radiance gathered after the vertex
+    const float* thp;        // [D][3][P] path throughput at the vertex
+    const float* pos;        // [D][3][P] world position
+    const float* dir;        // [D][2][P] scattered direction (theta, phi)
+    const float* wi_pdf;     // [D][P] effective pdf of the scattered direction
+    const float* mis_weight; // [D][P]
+    const uint8_t* delta;    // [D][P] sampled from a delta lobe
+    const float* bsdf_val;   // [D][3][P]
+    const float* wo;         // [D][2][P] outgoing direction (theta, phi)
+    const float* normal;     // [D][2][P] shading normal (theta, phi)
+    const float* roughness;  // [D][P]
+    const uint32_t* depth;   // [P] recorded vertices (curDepth)
+    uint32_t max_depth, n_pixels;
+};
+// Training pixels: thread t -> pixel (t % res[0], t / res[0]) * stride + offset.
+struct TrainTile {
+    uint32_t offset[2], stride[2], res[2], full[2];
+};
+hipError_t launch_pack_queries(const float* pos, const float* wo, const float* normal, const float* roughness,
+                               const SceneBox& box, uint32_t n, float* in, hipStream_t s);
+// Scratch: pack_training_scratch(tile) uint32 values.
+size_t pack_training_scratch(const TrainTile& t);
+hipError_t launch_pack_training(const PathRecords& r, const TrainTile& t, const SceneBox& box, uint32_t capacity,
+                                float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf, uint32_t* count,
+                                uint32_t* scratch, hipStream_t s);
+
 // FP8 inference: fragments with per-layer weight scales and the fused kernel's
 // multipliers (fp8 = wscale [L], ascale [L], qscale [L + 1]); calibration maxima.
 hipError_t launch_pack_frags_fp8(const DevCfg& c, const float* w, void* frag, const float* amax, float* fp8,

@@ -39,6 +39,16 @@ class NpmgQueue(ctypes.Structure):
                 ("out_pdf", _vp), ("out_eval_pdf", _vp), ("out_lobe", _vp), ("capacity", _u32)]
 
 
+class NpmgPathRecords(ctypes.Structure):
+    """npmg_path_records of include/npmg.h (device pointers, SoA per depth)."""
+    _fields_ = [(k, _vp) for k in ("L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo",
+                                   "normal", "roughness", "depth")] + [("max_depth", _u32), ("n_pixels", _u32)]
+
+
+class NpmgTrainTile(ctypes.Structure):
+    _fields_ = [("offset", _u32 * 2), ("stride", _u32 * 2), ("res", _u32 * 2), ("resolution", _u32 * 2)]
+
+
 # name -> (restype, argtypes); every function declared in include/npmg.h.
 SIGNATURES = {
     "npmg_create": (_int, [ctypes.c_char_p, _u64, ctypes.POINTER(_vp)]),
@@ -71,6 +81,9 @@ SIGNATURES = {
     "npmg_sample_pdf_queues": (_int, [_vp, ctypes.POINTER(NpmgQueue), _vp, _u32, _u32, _u32, _vp]),
     "npmg_autotune": (_int, [_vp, _u32, _vp]),
     "npmg_fp8_calibrate": (_int, [_vp, _vp, _u32, _vp]),
+    "npmg_pack_queries": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_f32), _u32, _vp, _vp]),
+    "npmg_pack_training": (_int, [_vp, ctypes.POINTER(NpmgPathRecords), ctypes.POINTER(NpmgTrainTile),
+                                  ctypes.POINTER(_f32), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 }
@@ -415,6 +428,27 @@ class GuidingNetwork:
         self._check(self._lib.npmg_fp8_scales(self._h, w, a, L))
         return list(w), list(a)
 
+    def pack_training(self, records, tile, box, capacity):
+        """generate_training_data: records = dict of device tensors named as the
+        npmg_path_records fields ([D][C][P] float, delta uint8, depth int32 [P]);
+        tile = dict(offset, stride, res, resolution) of pairs; box = 6 floats.
+        Returns (in [9][cap], tgt_dir [2][cap], tgt_rgb [3][cap], tgt_pdf [cap], count)."""
+        import torch
+
+        d = "cuda"
+        out = (torch.zeros(9, capacity, device=d), torch.zeros(2, capacity, device=d),
+               torch.zeros(3, capacity, device=d), torch.zeros(capacity, device=d))
+        count = torch.zeros(1, dtype=torch.int32, device=d)
+        r = NpmgPathRecords(*[_ptr(records[k]) for k in (
+            "L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo", "normal", "roughness",
+            "depth")], records["L"].shape[0], records["depth"].shape[0])
+        t = NpmgTrainTile((_u32 * 2)(*tile["offset"]), (_u32 * 2)(*tile["stride"]), (_u32 * 2)(*tile["res"]),
+                          (_u32 * 2)(*tile["resolution"]))
+        b = (_f32 * 6)(*box)
+        self._check(self._lib.npmg_pack_training(self._h, ctypes.byref(r), ctypes.byref(t), b, capacity,
+                                                 *[_ptr(o) for o in out], _ptr(count), _stream()))
+        return out + (count,)
+
     def tuning(self):
         c, b = _u32(), _u32()
         self._check(self._lib.npmg_tuning(self._h, ctypes.byref(c), ctypes.byref(b)))
@@ -469,6 +503,19 @@ def synth_train(seed: int, n: int, first: int = 0):
     if st != NPMG_OK:
         raise NpmgError(st, "npmg_synth_train failed")
     return t
+
+
+def pack_queries(pos, wo, normal, roughness, box):
+    """generate_inference_data: shading points [3][n] x 3 + roughness [n] -> in [9][n]."""
+    import torch
+
+    n = pos.shape[1]
+    out = torch.empty(9, n, device="cuda")
+    b = (_f32 * 6)(*box)
+    st = lib().npmg_pack_queries(_ptr(pos), _ptr(wo), _ptr(normal), _ptr(roughness), b, n, _ptr(out), _stream())
+    if st != NPMG_OK:
+        raise NpmgError(st, "npmg_pack_queries failed")
+    return out
 
 
 def debug_math(fn: int, x, y=None):

@@ -878,9 +878,112 @@ Head synth_head(Pcg32& r) {
     return h;
 }
 
+// ---- Wavefront packers ----------------------------------------------------
+
+// normalizeSpatialCoord (src/render/guided/train.h:173-177): the box inflated
+// by 0.5 % of its diagonal's length (AABB::inflate, krrmath/aabb.h:36-39),
+// then 0.5 + (x - center) / diagonal per axis (Eigen AlignedBox center and
+// diagonal).
+template <class T> void box_normalize(const T* lo, const T* hi, const T* x, T* out) {
+    const T dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    const T inflation = M<T>::sqrt(dx * dx + dy * dy + dz * dz) * (T)0.005;
+    for (int a = 0; a < 3; ++a) {
+        const T l = lo[a] - inflation, h = hi[a] + inflation;
+        const T center = (l + h) * (T)0.5, diag = h - l;
+        out[a] = (T)0.5 + (x[a] - center) / diag;
+    }
+}
+
+// One training record from recorded vertex (pid, d) (train.cu:29-64); false
+// when the reference drops it (delta vertex, NaN input/target, zero pdf).
+template <class T> struct PackRecords {
+    const T *L, *thp, *pos, *dir, *wi_pdf, *mis, *bsdf, *wo, *normal, *rough;
+    const uint8_t* delta;
+    size_t P;
+};
+template <class T>
+bool pack_train_record(const PackRecords<T>& r, const T* lo, const T* hi, size_t pid, size_t d, T* in, T* dir,
+                       T* rgb, T* pdf) {
+    const size_t P = r.P, s1 = d * P + pid, s2 = d * 2 * P + pid, s3 = d * 3 * P + pid;
+    if (r.delta[s1]) return false;
+    const T x[3] = {r.pos[s3], r.pos[s3 + P], r.pos[s3 + 2 * P]};
+    box_normalize(lo, hi, x, in);
+    const V3<T> w = sph_to_cart(r.wo[s2], r.wo[s2 + P]);  // warp_direction_for_sh (train.cu:6-8)
+    in[3] = (w.x + (T)1) * (T)0.5;
+    in[4] = (w.y + (T)1) * (T)0.5;
+    in[5] = (w.z + (T)1) * (T)0.5;
+    in[6] = r.normal[s2] * C<T>::inv_pi;
+    in[7] = r.normal[s2 + P] * C<T>::inv_2pi;
+    in[8] = (T)1 - M<T>::exp(-r.rough[s1]);  // warp_roughness_for_ob (train.cu:10-12)
+    bool nan_l = false;
+    for (int ch = 0; ch < 3; ++ch) {
+        const T thp = r.thp[s3 + ch * P];
+        T L = thp > C<T>::eps ? r.L[s3 + ch * P] / thp : (T)0;
+        L = L * r.mis[s1];
+        nan_l = nan_l || std::isnan(L);
+        rgb[ch] = L * r.bsdf[s3 + ch * P];  // output.L *= bsdfVal after the NaN test's L
+    }
+    dir[0] = r.dir[s2];
+    dir[1] = r.dir[s2 + P];
+    *pdf = r.wi_pdf[s1];
+    const bool bad = std::isnan(in[0]) || std::isnan(in[1]) || std::isnan(in[2]) || std::isnan(dir[0]) ||
+                     std::isnan(dir[1]) || std::isnan(in[3]) || std::isnan(in[4]) || std::isnan(in[5]) ||
+                     std::isnan(*pdf) || *pdf == (T)0 || nan_l;
+    return !bad;
+}
+
 }  // namespace
 
 extern "C" {
+
+void orc_pack_queries_f(const float* box, const float* pos, const float* wo, const float* normal,
+                        const float* roughness, int n, float* in) {
+    const size_t N = (size_t)n;
+    for (size_t i = 0; i < N; ++i) {
+        const float x[3] = {pos[i], pos[N + i], pos[2 * N + i]};
+        float p[3];
+        box_normalize(box, box + 3, x, p);
+        const V3<float> w = normalize(v3(wo[i], wo[N + i], wo[2 * N + i]));  // sd.wo.normalized()
+        float th, ph;
+        cart_to_sph(v3(normal[i], normal[N + i], normal[2 * N + i]), &th, &ph);
+        const float v[9] = {p[0], p[1], p[2], (w.x + 1.0f) * 0.5f, (w.y + 1.0f) * 0.5f, (w.z + 1.0f) * 0.5f,
+                            th / C<float>::pi, ph / C<float>::two_pi, 1.0f - fm::exp(-roughness[i])};
+        for (int d = 0; d < 9; ++d) in[(size_t)d * N + i] = v[d];
+    }
+}
+
+uint32_t orc_pack_training_f(const float* L, const float* thp, const float* pos, const float* dir,
+                             const float* wi_pdf, const float* mis, const uint8_t* delta, const float* bsdf,
+                             const float* wo, const float* normal, const float* rough, const uint32_t* depth,
+                             uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile, const float* box,
+                             uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf) {
+    const PackRecords<float> r{L, thp, pos, dir, wi_pdf, mis, bsdf, wo, normal, rough, delta, n_pixels};
+    // tile: offset xy, stride xy, training resolution xy, image resolution xy
+    const uint64_t n_threads = (uint64_t)tile[4] * tile[5];
+    uint32_t count = 0;
+    const size_t C = capacity;
+    for (uint64_t t = 0; t < n_threads; ++t) {
+        const uint32_t px = (uint32_t)(t % tile[4]) * tile[2] + tile[0];
+        const uint32_t py = (uint32_t)(t / tile[4]) * tile[3] + tile[1];
+        if (px >= tile[6] || py >= tile[7]) continue;
+        const size_t pid = (size_t)px + (size_t)py * tile[6];
+        const uint32_t nd = std::min(depth[pid], max_depth);
+        for (uint32_t d = 0; d < nd; ++d) {
+            float i9[9], dr[2], rgb[3], pdf;
+            if (!pack_train_record(r, box, box + 3, pid, d, i9, dr, rgb, &pdf)) continue;
+            if (count < capacity) {
+                for (int k = 0; k < 9; ++k) in[(size_t)k * C + count] = i9[k];
+                tgt_dir[count] = dr[0];
+                tgt_dir[C + count] = dr[1];
+                for (int k = 0; k < 3; ++k) tgt_rgb[(size_t)k * C + count] = rgb[k];
+                tgt_pdf[count] = pdf;
+            }
+            ++count;
+        }
+    }
+    return count;
+}
+
 
 int orc_encoded_width(const orc_cfg* c) { return enc_layout(c).padded; }
 int orc_padded_output(const orc_cfg* c) { return (c->n_output_dims + 15) / 16 * 16; }

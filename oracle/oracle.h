@@ -116,6 +116,17 @@ void orc_synth_queries(uint64_t seed, uint64_t first, int n, float* in, float* n
                        float* eval_dirs);
 void orc_synth_train(uint64_t seed, uint64_t first, int n, float* in, float* tgt_dir,
                      float* tgt_rgb, float* tgt_pdf);
+/* Wavefront packers: generate_inference_data (train.cu:174-191) and
+ * generate_training_data (train.cu:14-67) in (training pixel, depth) order.
+ * box: min xyz, max xyz; tile: offset xy, stride xy, training resolution xy,
+ * image resolution xy. Returns the number of records kept. */
+void orc_pack_queries_f(const float* box, const float* pos, const float* wo, const float* normal,
+                        const float* roughness, int n, float* in);
+uint32_t orc_pack_training_f(const float* L, const float* thp, const float* pos, const float* dir,
+                             const float* wi_pdf, const float* mis, const uint8_t* delta, const float* bsdf,
+                             const float* wo, const float* normal, const float* rough, const uint32_t* depth,
+                             uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile, const float* box,
+                             uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf);
 /* The analytic radiance field f(x, w) used for the training targets. */
 void orc_field_eval(uint64_t seed, const float* pos, const float* dir, int n, double* f);
 

@@ -122,6 +122,10 @@ class Oracle:
         L.orc_synth_train.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp, _vp]
         L.orc_field_eval.argtypes = [ctypes.c_uint64, _vp, _vp, ctypes.c_int, _vp]
         L.orc_init_params.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int, _vp]
+        L.orc_pack_queries_f.argtypes = [_vp] * 5 + [ctypes.c_int, _vp]
+        L.orc_pack_training_f.argtypes = [_vp] * 12 + [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
+                                                        ctypes.c_uint32] + [_vp] * 4
+        L.orc_pack_training_f.restype = ctypes.c_uint32
 
     # -- layout -----------------------------------------------------------
     def widths(self, cfg):
@@ -274,6 +278,32 @@ class Oracle:
         self.lib.orc_synth_train(seed, first, n, _p(a["inputs"]), _p(a["tgt_dir"]), _p(a["tgt_rgb"]),
                                  _p(a["tgt_pdf"]))
         return a
+
+    # -- wavefront packers -----------------------------------------------------
+    def pack_queries(self, box, pos, wo, normal, roughness):
+        k = _Keep()
+        n = pos.shape[1]
+        out = np.zeros((9, n), np.float32)
+        self.lib.orc_pack_queries_f(k(box, np.float32), k(pos, np.float32), k(wo, np.float32),
+                                    k(normal, np.float32), k(roughness, np.float32), n, _p(out))
+        return out
+
+    PACK_FIELDS = ("L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo", "normal",
+                   "roughness", "depth")
+
+    def pack_training(self, rec, tile, box, capacity):
+        """rec: dict of numpy arrays [D][C][P] (delta uint8, depth uint32 [P]);
+        tile: dict(offset, stride, res, resolution). Returns (in, dir, rgb, pdf, count)."""
+        k = _Keep()
+        ptrs = [k(rec[f], np.uint8 if f == "delta" else (np.uint32 if f == "depth" else np.float32))
+                for f in self.PACK_FIELDS]
+        t = np.array(list(tile["offset"]) + list(tile["stride"]) + list(tile["res"]) + list(tile["resolution"]),
+                     np.uint32)
+        out = (np.zeros((9, capacity), np.float32), np.zeros((2, capacity), np.float32),
+               np.zeros((3, capacity), np.float32), np.zeros(capacity, np.float32))
+        cnt = self.lib.orc_pack_training_f(*ptrs, rec["L"].shape[0], rec["depth"].shape[0], k(t, np.uint32),
+                                           k(box, np.float32), capacity, *[_p(o) for o in out])
+        return out + (int(cnt),)
 
     def field_eval(self, seed, pos, dirs):
         k = _Keep()
