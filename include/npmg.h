@@ -172,6 +172,24 @@ npmg_status npmg_fp8_calibrate(npmg_ctx* ctx, const float* in, uint32_t n, npmg_
  * layer l computes (W_l sw_l)(x_l sx_l) in E4M3. Synchronous. */
 npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_scales, uint32_t n_layers);
 
+/* ---- Snapshots ------------------------------------------------------------ */
+
+/* Checkpoint in tcnn's snapshot format: the MessagePack document that
+ * nlohmann::json::to_msgpack writes for Trainer::serialize (trainer.h:267-278):
+ * n_params, params_type "float", params_binary (the inference parameters) and,
+ * with with_optimizer, optimizer = {nested: Adam state (adam.h:282-290:
+ * current_step, base_learning_rate, first/second moments, param steps),
+ * weights_ema_binary} (ema.h:194-199; Adam alone without EMA). An extra
+ * "npmg" map holds the FP32 master weights and the loss-scale state, so that
+ * a resume is exact; tcnn ignores it. Host buffer: call with buf = NULL to get
+ * the size, then with a buffer of at least that size. Synchronous. */
+npmg_status npmg_snapshot_save(npmg_ctx* ctx, int with_optimizer, void* buf, size_t capacity, size_t* size);
+/* Restores a snapshot written by npmg_snapshot_save or by tcnn (params_type
+ * "float" or "__half"; MessagePack). Without "npmg", the master and EMA
+ * weights both take params_binary (Trainer::deserialize, trainer.h:281-307);
+ * without "optimizer" the optimiser state is left as it is. Synchronous. */
+npmg_status npmg_snapshot_load(npmg_ctx* ctx, const void* buf, size_t size);
+
 /* ---- Wavefront packers (either side of the path) ------------------------ */
 
 /* generate_inference_data (train.cu:174-191): shading points of a queue ->

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/npmg.h"
+#include "msgpack.h"
 #include "config.h"
 #include "kernels.h"
 #include "npmg_math.h"
@@ -974,6 +975,200 @@ npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream) {
     return guarded(ctx, [&] {
         ctx->check_fused();
         ctx->autotune(n, S(stream));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// IEEE binary16 -> binary32 (tcnn's "__half" snapshots).
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31, exp = (h >> 10) & 0x1fu, man = h & 0x3ffu;
+    uint32_t bits;
+    if (exp == 0x1fu) bits = sign | 0x7f800000u | (man << 13);
+    else if (exp != 0) bits = sign | ((exp + 112u) << 23) | (man << 13);
+    else if (man == 0) bits = sign;
+    else {  // subnormal: normalise
+        int e = -1;
+        uint32_t m = man;
+        do {
+            ++e;
+            m <<= 1;
+        } while (!(m & 0x400u));
+        bits = sign | ((uint32_t)(112 - e) << 23) | ((m & 0x3ffu) << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// n float values from a "float" (4 n bytes) or "__half" (2 n bytes) blob.
+std::vector<float> blob_floats(const npmg::mp::Value* v, size_t n, const char* what) {
+    if (!v || v->kind != npmg::mp::Value::BIN) throw std::invalid_argument(std::string("snapshot: missing ") + what);
+    std::vector<float> out(n);
+    if (v->bn == n * 4) {
+        std::memcpy(out.data(), v->b, n * 4);
+    } else if (v->bn == n * 2) {
+        for (size_t i = 0; i < n; ++i) {
+            uint16_t h;
+            std::memcpy(&h, v->b + 2 * i, 2);
+            out[i] = half_to_float(h);
+        }
+    } else {
+        throw std::invalid_argument(std::string("snapshot: ") + what + " has the wrong size");
+    }
+    return out;
+}
+
+// tcnn Trainer::serialize (trainer.h:267-278) with the optimizer state of
+// Adam (adam.h:282-290) nested in Ema (ema.h:194-199), keys in nlohmann's
+// sorted order, plus an "npmg" map (FP32 master weights and the loss-scale
+// state) that tcnn ignores and that makes a resume exact. With dry, blobs are
+// counted instead of written.
+std::vector<uint8_t> build_snapshot(const npmg_ctx* ctx, bool with_optimizer, bool dry, size_t* total) {
+    const Config& c = ctx->cfg;
+    const size_t n = c.n_params;
+    npmg::mp::Writer w;
+    size_t skipped = 0;
+    auto blob = [&](const void* dev, size_t bytes) {
+        if (dry) {
+            w.bin_header(bytes);
+            skipped += bytes;
+            return;
+        }
+        std::vector<uint8_t> h(bytes);
+        hip_check(hipMemcpy(h.data(), dev, bytes, hipMemcpyDeviceToHost), "download");
+        w.bin(h.data(), bytes);
+    };
+    ScaleState st{};
+    if (!dry) hip_check(hipMemcpy(&st, ctx->sstate, sizeof(st), hipMemcpyDeviceToHost), "download");
+    w.map(with_optimizer ? 5u : 4u);
+    w.str("n_params");
+    w.uint(n);
+    w.str("npmg");
+    w.map(3);
+    w.str("loss_scale");
+    w.f64(st.scale);
+    w.str("master_binary");
+    blob(ctx->master, n * 4);
+    w.str("scale_good");
+    w.uint(st.good);
+    if (with_optimizer) {
+        w.str("optimizer");
+        auto adam = [&] {
+            w.map(5);
+            w.str("base_learning_rate");
+            w.f64(c.lr);
+            w.str("current_step");
+            w.uint(st.step);
+            w.str("first_moments_binary");
+            blob(ctx->m, n * 4);
+            w.str("param_steps_binary");
+            blob(ctx->steps, n * 4);
+            w.str("second_moments_binary");
+            blob(ctx->v, n * 4);
+        };
+        if (ctx->ema) {
+            w.map(2);
+            w.str("nested");
+            adam();
+            w.str("weights_ema_binary");
+            blob(ctx->ema, n * 4);
+        } else {
+            adam();
+        }
+    }
+    w.str("params_binary");
+    blob(ctx->infer_params(), n * 4);
+    w.str("params_type");
+    w.str("float");
+    *total = w.out.size() + skipped;
+    return std::move(w.out);
+}
+
+}  // namespace
+
+extern "C" {
+
+npmg_status npmg_snapshot_save(npmg_ctx* ctx, int with_optimizer, void* buf, size_t capacity, size_t* size) {
+    if (!ctx || !size) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        size_t total = 0;
+        if (!buf) {
+            build_snapshot(ctx, with_optimizer != 0, true, &total);
+            *size = total;
+            return;
+        }
+        std::vector<uint8_t> doc = build_snapshot(ctx, with_optimizer != 0, false, &total);
+        *size = total;
+        need(capacity >= total, "snapshot buffer too small (query the size with buf = NULL)");
+        std::memcpy(buf, doc.data(), total);
+    });
+}
+
+npmg_status npmg_snapshot_load(npmg_ctx* ctx, const void* buf, size_t size) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(buf && size, "empty snapshot");
+        const npmg::mp::Value doc = npmg::mp::Reader(static_cast<const uint8_t*>(buf), size).parse();
+        need(doc.kind == npmg::mp::Value::MAP, "snapshot: not a map");
+        const size_t n = ctx->cfg.n_params;
+        const npmg::mp::Value* np = doc.get("n_params");
+        need(np && np->is_number() && (size_t)np->number() == n, "snapshot: n_params differs from this network");
+        const npmg::mp::Value* ty = doc.get("params_type");
+        if (ty) need(ty->kind == npmg::mp::Value::STR && (ty->s == "float" || ty->s == "__half"),
+                     "snapshot: params_type must be float or __half");
+        const std::vector<float> params = blob_floats(doc.get("params_binary"), n, "params_binary");
+        const npmg::mp::Value* ext = doc.get("npmg");
+        const std::vector<float> master =
+            ext && ext->get("master_binary") ? blob_floats(ext->get("master_binary"), n, "master_binary") : params;
+        // optimizer state (all of it parsed before anything is written)
+        const npmg::mp::Value* opt = doc.get("optimizer");
+        std::vector<float> ema, m, v;
+        std::vector<uint32_t> steps;
+        uint32_t cur_step = 0;
+        if (opt) {
+            need(opt->kind == npmg::mp::Value::MAP, "snapshot: optimizer must be a map");
+            const npmg::mp::Value* adam = opt->get("nested") ? opt->get("nested") : opt;
+            if (ctx->ema) ema = opt->get("weights_ema_binary") ? blob_floats(opt->get("weights_ema_binary"), n,
+                                                                             "weights_ema_binary")
+                                                                 : params;
+            m = blob_floats(adam->get("first_moments_binary"), n, "first_moments_binary");
+            v = blob_floats(adam->get("second_moments_binary"), n, "second_moments_binary");
+            steps.assign(n, 0u);
+            if (const npmg::mp::Value* ps = adam->get("param_steps_binary")) {
+                need(ps->kind == npmg::mp::Value::BIN && ps->bn == n * 4, "snapshot: param_steps_binary size");
+                std::memcpy(steps.data(), ps->b, n * 4);
+            }
+            const npmg::mp::Value* cs = adam->get("current_step");
+            need(cs && cs->is_number() && cs->number() >= 0, "snapshot: current_step");
+            cur_step = (uint32_t)cs->number();
+        } else if (ctx->ema) {
+            ema = params;  // set_params_full_precision sets every copy (trainer.h:288)
+        }
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        hip_check(hipMemcpy(ctx->master, master.data(), n * 4, hipMemcpyHostToDevice), "upload");
+        if (ctx->ema) hip_check(hipMemcpy(ctx->ema, ema.data(), n * 4, hipMemcpyHostToDevice), "upload");
+        ScaleState st{};
+        hip_check(hipMemcpy(&st, ctx->sstate, sizeof(st), hipMemcpyDeviceToHost), "download");
+        if (opt) {
+            hip_check(hipMemcpy(ctx->m, m.data(), n * 4, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(ctx->v, v.data(), n * 4, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(ctx->steps, steps.data(), n * 4, hipMemcpyHostToDevice), "upload");
+            st.step = cur_step;
+        }
+        if (ext) {
+            const npmg::mp::Value* ls = ext->get("loss_scale");
+            const npmg::mp::Value* gd = ext->get("scale_good");
+            if (ls && ls->is_number() && ls->number() > 0) st.scale = (float)ls->number();
+            if (gd && gd->is_number()) st.good = (uint32_t)gd->number();
+        }
+        st.overflow = 0u;
+        hip_check(hipMemcpy(ctx->sstate, &st, sizeof(st), hipMemcpyHostToDevice), "upload");
+        ctx->refresh_layouts(nullptr);
+        hip_check(hipDeviceSynchronize(), "synchronize");
     });
 }
 

@@ -81,6 +81,8 @@ SIGNATURES = {
     "npmg_sample_pdf_queues": (_int, [_vp, ctypes.POINTER(NpmgQueue), _vp, _u32, _u32, _u32, _vp]),
     "npmg_autotune": (_int, [_vp, _u32, _vp]),
     "npmg_fp8_calibrate": (_int, [_vp, _vp, _u32, _vp]),
+    "npmg_snapshot_save": (_int, [_vp, _int, _vp, _sz, ctypes.POINTER(_sz)]),
+    "npmg_snapshot_load": (_int, [_vp, _vp, _sz]),
     "npmg_pack_queries": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_f32), _u32, _vp, _vp]),
     "npmg_pack_training": (_int, [_vp, ctypes.POINTER(NpmgPathRecords), ctypes.POINTER(NpmgTrainTile),
                                   ctypes.POINTER(_f32), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -448,6 +450,18 @@ class GuidingNetwork:
         self._check(self._lib.npmg_pack_training(self._h, ctypes.byref(r), ctypes.byref(t), b, capacity,
                                                  *[_ptr(o) for o in out], _ptr(count), _stream()))
         return out + (count,)
+
+    def snapshot(self, with_optimizer: bool = True) -> bytes:
+        """tcnn-format MessagePack snapshot (Trainer::serialize) as bytes."""
+        size = _sz()
+        self._check(self._lib.npmg_snapshot_save(self._h, int(with_optimizer), None, 0, ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(size.value)
+        self._check(self._lib.npmg_snapshot_save(self._h, int(with_optimizer), buf, size.value, ctypes.byref(size)))
+        return buf.raw[:size.value]
+
+    def load_snapshot(self, data: bytes):
+        buf = ctypes.create_string_buffer(data, len(data))
+        self._check(self._lib.npmg_snapshot_load(self._h, buf, len(data)))
 
     def tuning(self):
         c, b = _u32(), _u32()
