@@ -133,16 +133,18 @@ def main():
     lib = npmg.lib()
     P = npmg._ptr
 
-    def full_cfg(precision):
-        """The full configuration; "fp8" is BF16 training with FP8 E4M3 inference."""
+    def full_cfg(precision, numerics="fast"):
+        """The full configuration; "fp8" is BF16 training with FP8 E4M3 inference.
+        Every precision's mixture stage uses the same numerics ("fast": hardware
+        transcendentals; "stable": the bit-exact functions of the FP32 parity path)."""
+        c = npmg.benchmark_config("full", precision="bf16" if precision == "fp8" else precision)
         if precision == "fp8":
-            c = npmg.benchmark_config("full", precision="bf16")
             c["inference_precision"] = "fp8"
-            return c
-        return npmg.benchmark_config("full", precision=precision)
+        c["mixture"] = dict(c["mixture"], numerics=numerics)
+        return c
 
-    def measure_queries(precision):
-        net = npmg.GuidingNetwork(full_cfg(precision), seed=7272)
+    def measure_queries(precision, numerics="fast"):
+        net = npmg.GuidingNetwork(full_cfg(precision, numerics), seed=7272)
 
         def query_step():
             st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
@@ -152,6 +154,10 @@ def main():
                 raise npmg.NpmgError(st, lib.npmg_last_error(net._h).decode())
 
         ms = time_region(query_step, args.steps, max(1, args.warmup), stream)  # FP8: the first call calibrates
+        if numerics != "fast":
+            log(f"{precision} ({numerics} numerics) infer+sample+pdf: {ms:.3f} ms per 2^21 queries")
+            net.close()
+            return None, ms
         log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
         # per-kernel launch durations: HIP events recorded by the library around each
         # launch, on the stream that launch runs on (a separate pass, not the timed one)
@@ -291,6 +297,7 @@ def main():
             netp.close()
         else:
             net = netp
+    ms_exact = {p: measure_queries(p, "stable")[1] for p in (["fp32"] + ([args.precision] if args.precision != "fp32" else []))}
     log(f"network: {net.n_mlp} MLP + {net.n_grid} grid parameters")
     ms_q = per_prec[args.precision]
     qps = N_QUERIES / (ms_q * 1e-3)
@@ -326,7 +333,7 @@ def main():
         "dtype": args.precision,
         "data": "synthetic (pcg32-keyed hit records, analytic radiance field)",
         "config": {"workload": "infer+sample+pdf of 2^21 queries, hash grid 16x2^19x2 + OneBlob(16)x6, "
-                               "MLP 3x64, 8 vMF lobes, cosine product, 4 eval directions",
+                               "MLP 3x64, 8 vMF lobes, cosine product, 4 eval directions, fast mixture numerics",
                    "queries_per_step": N_QUERIES, "train_batch": N_TRAIN, "precision": args.precision},
         "train_samples_per_s": sps,
         "train_ms_per_step": ms_t,
@@ -341,6 +348,9 @@ def main():
                         "achieved_tflops": flops / (kq["avg_ms_per_launch"] * 1e-3) / 1e12,
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
         "queries_per_s_by_precision": {k: N_QUERIES / (v * 1e-3) for k, v in per_prec.items()},
+        "numerics": "fast (hardware transcendentals in the mixture stage, pdfs within 2e-5 of FP64 on the same "
+                    "network outputs); the bit-exact stable numerics beside it",
+        "queries_per_s_stable_numerics": {k: N_QUERIES / (v * 1e-3) for k, v in ms_exact.items()},
         "train_samples_per_s_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_train.items()},
         "train_samples_per_s_graph_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_graph.items()},
         "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16")},

@@ -165,7 +165,8 @@ Config parse_config(const std::string& text) {
     const std::string num = mix.kind == Json::Object ? mix.string("numerics", "stable") : "stable";
     if (ieq(num, "reference")) d.numerics = NUM_REFERENCE;
     else if (ieq(num, "stable")) d.numerics = NUM_STABLE;
-    else bad("mixture numerics must be reference or stable");
+    else if (ieq(num, "fast")) d.numerics = NUM_FAST;
+    else bad("mixture numerics must be reference, stable or fast");
     d.cos_kappa = mix.kind == Json::Object ? (float)mix.number("cosine_kappa", 2.18853) : 2.18853f;
     d.n_out = d.lobe_params * d.K;
     d.n_out_pad = (d.n_out + 15) / 16 * 16;

@@ -13,7 +13,9 @@ enum GridType : int { GRID_HASH = 0, GRID_DENSE = 1 };
 enum HashType : int { HASH_COHERENT_PRIME = 0, HASH_PRIME = 1, HASH_REVERSED_PRIME = 2 };
 enum Interp : int { INTERP_LINEAR = 0, INTERP_SMOOTHSTEP = 1, INTERP_NEAREST = 2 };
 enum Mapping : int { MAP_REFERENCE = 0, MAP_SOFTMAX = 1 };
-enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1 };
+// fast: the stable formulas with hardware transcendentals on the query path
+// (the training adjoint keeps the exact functions).
+enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1, NUM_FAST = 2 };
 enum MixtureType : int { MIX_VMF = 0, MIX_ASG = 1 };
 enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3 };
 

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "config.h"
+#include "kernels.h"
 #include "npmg_math.h"
 
 namespace npmg {
@@ -41,16 +42,52 @@ __device__ __forceinline__ V3 cross(V3 a, V3 b) {
 }
 __device__ __forceinline__ float length(V3 a) { return fm::sqrt_(dot(a, a)); }
 __device__ __forceinline__ V3 normalize(V3 a) { return divs(a, length(a)); }
+
+// Math policies of the mixture stage. ExactMath is the library's own
+// deterministic FP32 functions (npmg_math.h), bit-identical to the oracle's
+// FP32 instantiation. FastMath ("numerics": "fast") uses the hardware
+// transcendentals (v_exp_f32 / v_log_f32 base 2, v_sin_f32 / v_cos_f32 in
+// revolutions, v_rcp_f32, v_sqrt_f32: about 1 ulp each) for exp, log,
+// sin/cos, division and sqrt; expm1 and log1p stay exact (their arguments
+// can be small).
+struct ExactMath {
+    static __device__ __forceinline__ float exp(float x) { return fm::exp(x); }
+    static __device__ __forceinline__ float log(float x) { return fm::log(x); }
+    static __device__ __forceinline__ float expm1(float x) { return fm::expm1(x); }
+    static __device__ __forceinline__ float log1p(float x) { return fm::log1p(x); }
+    static __device__ __forceinline__ void sincos(float x, float* s, float* c) { fm::sincos(x, s, c); }
+    static __device__ __forceinline__ float sqrt(float x) { return fm::sqrt_(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+};
+struct FastMath {
+    static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+    static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+    static __device__ __forceinline__ float expm1(float x) { return fm::expm1(x); }
+    static __device__ __forceinline__ float log1p(float x) { return fm::log1p(x); }
+    static __device__ __forceinline__ void sincos(float x, float* s, float* c) {
+        const float r = x * 0.159154943091895336f;  // revolutions
+        *s = __builtin_amdgcn_sinf(r);
+        *c = __builtin_amdgcn_cosf(r);
+    }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+template <class MT>
+__device__ __forceinline__ V3 normalize_m(V3 a) {
+    if constexpr (__is_same(MT, ExactMath)) return normalize(a);
+    else return mul(a, __builtin_amdgcn_rsqf(dot(a, a)));
+}
 // std::min / std::max semantics (second argument wins only when strictly smaller / larger).
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float clampv(float v, float lo, float hi) { return smin(smax(v, lo), hi); }
 
 // math_utils.h:191-201
+template <class MT = ExactMath>
 __device__ __forceinline__ V3 sph_to_cart(float theta, float phi) {
     float st, ct, sp, cp;
-    fm::sincos(theta, &st, &ct);
-    fm::sincos(phi, &sp, &cp);
+    MT::sincos(theta, &st, &ct);
+    MT::sincos(phi, &sp, &cp);
     return {st * cp, st * sp, ct};
 }
 // math_utils.h:176-183
@@ -62,6 +99,7 @@ __device__ __forceinline__ void cart_to_sph(V3 v, float* theta, float* phi) {
     *phi = p;
 }
 // math_utils.h:141-151
+template <class MT = ExactMath>
 __device__ __forceinline__ V3 get_perpendicular(V3 u) {
     const float ax = fm::abs_(u.x), ay = fm::abs_(u.y), az = fm::abs_(u.z);
     const uint32_t uyx = (ax - ay) < 0 ? 1u : 0u;
@@ -70,20 +108,22 @@ __device__ __forceinline__ V3 get_perpendicular(V3 u) {
     const uint32_t xm = uyx & uzx;
     const uint32_t ym = (1u ^ xm) & uzy;
     const uint32_t zm = 1u ^ (xm | ym);
-    return normalize(cross(u, mk((float)xm, (float)ym, (float)zm)));
+    return normalize_m<MT>(cross(u, mk((float)xm, (float)ym, (float)zm)));
 }
 // raytracing.h:35-56: Frame(n).toWorld(v)
+template <class MT = ExactMath>
 __device__ __forceinline__ V3 frame_to_world(V3 n, V3 v) {
-    const V3 t = get_perpendicular(n);
-    const V3 b = normalize(cross(n, t));
+    const V3 t = get_perpendicular<MT>(n);
+    const V3 b = normalize_m<MT>(cross(n, t));
     return add(add(mul(t, v.x), mul(b, v.y)), mul(n, v.z));
 }
 // sampling.h:32-37
+template <class MT = ExactMath>
 __device__ __forceinline__ V3 uniform_sample_sphere(float u0, float u1) {
     const float z = 1.0f - 2.0f * u0;
-    const float r = fm::sqrt_(smax(0.0f, 1.0f - z * z));
+    const float r = MT::sqrt(smax(0.0f, 1.0f - z * z));
     float s, c;
-    fm::sincos(kTwoPi * u1, &s, &c);
+    MT::sincos(kTwoPi * u1, &s, &c);
     return {r * c, r * s, z};
 }
 
@@ -315,9 +355,10 @@ struct Mixture {
 // The vMF normaliser with the e^{-kappa} of the exponent folded in,
 // C(k) = k e^{-k} / (4 pi sinh k) = k / (2 pi (-expm1(-2 k))), exactly 1/4pi at
 // k = 0; the pdf is C(k) exp(k (mu.w - 1)), evaluated as exp(k (mu.w - 1) + log C).
+template <class MT = ExactMath>
 __device__ __forceinline__ float stable_norm(float k) {
     if (!(k > 1e-30f)) return kInv4Pi;
-    return k * kInv2Pi / -fm::expm1(-2.0f * k);
+    return MT::div(k * kInv2Pi, -MT::expm1(-2.0f * k));
 }
 // Reference normaliser (vmf.h:22-27 and distribution.h:40-42) with its 1/4pi threshold.
 __device__ __forceinline__ float ref_norm(float k, float thresh) {
@@ -328,17 +369,17 @@ __device__ __forceinline__ float ref_norm(float k, float thresh) {
 // x points at the first raw output of the query; `stride` is the distance
 // between consecutive outputs. Mapping "softmax" reads (theta, phi, log
 // kappa, logit) per lobe.
-template <int K, class Raw>
+template <int K, class MT = ExactMath, class Raw>
 __device__ __forceinline__ void build_mixture(const DevCfg& c, Raw&& x, Mixture<K>& m) {
     m.total = 0.0f;
     float theta[K], phi[K];
     if (c.mapping == MAP_REFERENCE) {
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            m.lambda[i] = fm::exp(clampv(x(4 * i + 0), -10.0f, 15.0f));
-            m.kappa[i] = fm::exp(clampv(x(4 * i + 1), -10.0f, 15.0f));
-            theta[i] = kPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 2))));
-            phi[i] = kTwoPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 3))));
+            m.lambda[i] = MT::exp(clampv(x(4 * i + 0), -10.0f, 15.0f));
+            m.kappa[i] = MT::exp(clampv(x(4 * i + 1), -10.0f, 15.0f));
+            theta[i] = kPi * MT::div(1.0f, 1.0f + MT::exp(-x(4 * i + 2)));
+            phi[i] = kTwoPi * MT::div(1.0f, 1.0f + MT::exp(-x(4 * i + 3)));
             m.total += m.lambda[i];
         }
     } else {
@@ -347,43 +388,44 @@ __device__ __forceinline__ void build_mixture(const DevCfg& c, Raw&& x, Mixture<
         for (int i = 1; i < K; ++i) mx = smax(mx, x(4 * i + 3));
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            theta[i] = kPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 0))));
-            phi[i] = kTwoPi * (1.0f / (1.0f + fm::exp(-x(4 * i + 1))));
-            m.kappa[i] = clampv(fm::exp(x(4 * i + 2)), 1e-2f, 1e3f);
-            m.lambda[i] = fm::exp(x(4 * i + 3) - mx);
+            theta[i] = kPi * MT::div(1.0f, 1.0f + MT::exp(-x(4 * i + 0)));
+            phi[i] = kTwoPi * MT::div(1.0f, 1.0f + MT::exp(-x(4 * i + 1)));
+            m.kappa[i] = clampv(MT::exp(x(4 * i + 2)), 1e-2f, 1e3f);
+            m.lambda[i] = MT::exp(x(4 * i + 3) - mx);
             m.total += m.lambda[i];
         }
     }
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-        m.weight[i] = m.lambda[i] / m.total;
-        m.mu[i] = sph_to_cart(theta[i], phi[i]);
-        m.norm[i] = stable_norm(m.kappa[i]);
-        m.lognorm[i] = fm::log(m.norm[i]);
+        m.weight[i] = MT::div(m.lambda[i], m.total);
+        m.mu[i] = sph_to_cart<MT>(theta[i], phi[i]);
+        m.norm[i] = stable_norm<MT>(m.kappa[i]);
+        m.lognorm[i] = MT::log(m.norm[i]);
     }
 }
 
 // applyCosineLobe (distribution.h:184-195) with VMFKernel::product (:39-64).
 // Reference numerics take each new mean back through spherical coordinates
 // like the reference; stable numerics keep the normalised cartesian mean.
-template <int K>
+template <int K, class MT = ExactMath>
 __device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>& m, bool want_angles,
                                              float (&theta_out)[K], float (&phi_out)[K]) {
     const bool ref = c.numerics == NUM_REFERENCE;
     float nt, np;
     cart_to_sph(nrm, &nt, &np);
     const float kb = c.cos_kappa;
-    const V3 mu_b = sph_to_cart(nt, np);
-    const float norm_b = ref ? ref_norm(kb, 1e-3f) : stable_norm(kb);
+    const V3 mu_b = sph_to_cart<MT>(nt, np);
+    const float norm_b = ref ? ref_norm(kb, 1e-3f) : stable_norm<MT>(kb);
     m.total = 0.0f;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const V3 mu_a = m.mu[i];
         const float ka = m.kappa[i];
         V3 nm = add(mul(mu_a, ka), mul(mu_b, kb));
-        float nk = length(nm);
+        float nk = MT::sqrt(dot(nm, nm));
         if (nk >= 1e-3f) {
-            nm = divs(nm, nk);
+            if constexpr (__is_same(MT, ExactMath)) nm = divs(nm, nk);
+            else nm = mul(nm, MT::div(1.0f, nk));
         } else {
             nk = 0.0f;
             nm = mu_a;
@@ -396,25 +438,25 @@ __device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>
             theta_out[i] = th;
             phi_out[i] = ph;
         }
-        const float e = fm::exp(ka * (dot(nm, mu_a) - 1.0f) + kb * (dot(nm, mu_b) - 1.0f));
+        const float e = MT::exp(ka * (dot(nm, mu_a) - 1.0f) + kb * (dot(nm, mu_b) - 1.0f));
         const float na = ref ? ref_norm(ka, 1e-3f) : m.norm[i];
-        const float cn = stable_norm(nk);
+        const float cn = stable_norm<MT>(nk);
         const float nn = ref ? ref_norm(nk, 1e-3f) : cn;
-        const float s = na * norm_b / nn * e;
+        const float s = MT::div(na * norm_b, nn) * e;
         m.lambda[i] = s * m.lambda[i] * 1.0f;
         m.kappa[i] = nk;
         m.mu[i] = mu_new;
         m.norm[i] = cn;
-        m.lognorm[i] = fm::log(cn);
+        m.lognorm[i] = MT::log(cn);
         m.total += m.lambda[i];
     }
     m.total = smax(m.total, kEps);
 #pragma unroll
-    for (int i = 0; i < K; ++i) m.weight[i] = m.lambda[i] / m.total;
+    for (int i = 0; i < K; ++i) m.weight[i] = MT::div(m.lambda[i], m.total);
 }
 
 // One lobe's pdf at unit direction w (vmf.h:22-27, or the log-space form).
-template <int K>
+template <int K, class MT = ExactMath>
 __device__ __forceinline__ float lobe_pdf(const DevCfg& c, const Mixture<K>& m, int i, V3 w) {
     const float cth = dot(w, m.mu[i]);
     const float d = smin(0.0f, cth - 1.0f);
@@ -423,21 +465,21 @@ __device__ __forceinline__ float lobe_pdf(const DevCfg& c, const Mixture<K>& m, 
         if (k < kEps) return kInv4Pi;
         return fm::exp(k * d) * k / (kTwoPi * (1.0f - fm::exp(-2.0f * k)));
     }
-    return fm::exp(fm::fma_(m.kappa[i], d, m.lognorm[i]));
+    return MT::exp(fm::fma_(m.kappa[i], d, m.lognorm[i]));
 }
 
 // distribution.h:113-119
-template <int K>
+template <int K, class MT = ExactMath>
 __device__ __forceinline__ float mixture_pdf(const DevCfg& c, const Mixture<K>& m, V3 w) {
     float p = 0.0f;
 #pragma unroll
-    for (int i = 0; i < K; ++i) p = fm::fma_(m.weight[i], lobe_pdf(c, m, i, w), p);
+    for (int i = 0; i < K; ++i) p = fm::fma_(m.weight[i], lobe_pdf<K, MT>(c, m, i, w), p);
     return p;
 }
 
 // distribution.h:132-140 (lobe choice by running subtraction, fallback lobe 0)
 // and vmf.h:48-71 (inverse-CDF sample about +z rotated into Frame(mu)).
-template <int K>
+template <int K, class MT = ExactMath>
 __device__ __forceinline__ V3 mixture_sample(const DevCfg& c, const Mixture<K>& m, float u0, float u1, float u2,
                                              uint32_t* lobe) {
     int sel = -1;
@@ -468,17 +510,70 @@ __device__ __forceinline__ V3 mixture_sample(const DevCfg& c, const Mixture<K>& 
     const bool ref = c.numerics == NUM_REFERENCE;
     V3 loc;
     if (ref ? kap < kEps : !(kap > 1e-30f)) {
-        loc = uniform_sample_sphere(u1, u2);
+        loc = uniform_sample_sphere<MT>(u1, u2);
     } else {
         float ct;
         if (ref) ct = 1.0f + fm::log1p(-u1 + fm::exp(-2.0f * kap) * u1) / kap;
-        else ct = 1.0f + fm::log1p(u1 * fm::expm1(-2.0f * kap)) / kap;
-        const float st = fm::sqrt_(smax(0.0f, 1.0f - ct * ct));
+        else ct = 1.0f + MT::div(fm::log1p(u1 * fm::expm1(-2.0f * kap)), kap);
+        const float st = MT::sqrt(smax(0.0f, 1.0f - ct * ct));
         float sp, cp;
-        fm::sincos(kTwoPi * u2, &sp, &cp);
+        MT::sincos(kTwoPi * u2, &sp, &cp);
         loc = {cp * st, sp * st, ct};
     }
-    return frame_to_world(mu, loc);
+    return frame_to_world<MT>(mu, loc);
+}
+
+// The consumer side of one query (integrator.cpp:197-201, 256-260, 297-317):
+// the mixture from raw outputs x, the cosine product when io.normal is set,
+// one sample with its pdf and the pdfs of io.n_eval directions, and the lobe
+// parameters when io.lobes_out is set (LOBES: the consumer kernel only).
+template <int K, class MT, bool LOBES, class Raw>
+__device__ __forceinline__ void vmf_query(const DevCfg& c, Raw&& x, const QueryIO& io, uint32_t q, uint32_t n) {
+    Mixture<K> m;
+    build_mixture<K, MT>(c, x, m);
+    float th[K], ph[K];
+    if (io.normal) {
+        const V3 nrm = mk(io.normal[q], io.normal[(size_t)n + q], io.normal[2 * (size_t)n + q]);
+        apply_cosine<K, MT>(c, nrm, m, LOBES && io.lobes_out != nullptr, th, ph);
+    } else if (LOBES && io.lobes_out) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const bool r = c.mapping == MAP_REFERENCE;
+            th[i] = kPi * MT::div(1.0f, 1.0f + MT::exp(-(r ? x(4 * i + 2) : x(4 * i))));
+            ph[i] = kTwoPi * MT::div(1.0f, 1.0f + MT::exp(-(r ? x(4 * i + 3) : x(4 * i + 1))));
+        }
+    }
+    if (LOBES && io.lobes_out) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            io.lobes_out[(size_t)(5 * i + 0) * n + q] = m.lambda[i];
+            io.lobes_out[(size_t)(5 * i + 1) * n + q] = m.kappa[i];
+            io.lobes_out[(size_t)(5 * i + 2) * n + q] = th[i];
+            io.lobes_out[(size_t)(5 * i + 3) * n + q] = ph[i];
+            io.lobes_out[(size_t)(5 * i + 4) * n + q] = m.weight[i];
+        }
+    }
+    if (io.u) {
+        uint32_t lobe;
+        const V3 d = mixture_sample<K, MT>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
+        io.out_dir[q] = d.x;
+        io.out_dir[(size_t)n + q] = d.y;
+        io.out_dir[2 * (size_t)n + q] = d.z;
+        io.out_pdf[q] = mixture_pdf<K, MT>(c, m, d);
+        if (io.out_lobe) io.out_lobe[q] = lobe;
+    }
+    for (int e = 0; e < io.n_eval; ++e) {
+        const float* ed = io.eval_dirs + (size_t)e * 3 * n;
+        const V3 w = mk(ed[q], ed[(size_t)n + q], ed[2 * (size_t)n + q]);
+        io.out_eval_pdf[(size_t)e * n + q] = mixture_pdf<K, MT>(c, m, w);
+    }
+}
+
+// vmf_query with the math policy of the configuration's numerics.
+template <int K, bool LOBES, class Raw>
+__device__ __forceinline__ void vmf_query_cfg(const DevCfg& c, Raw&& x, const QueryIO& io, uint32_t q, uint32_t n) {
+    if (c.numerics == NUM_FAST) vmf_query<K, FastMath, LOBES>(c, x, io, q, n);
+    else vmf_query<K, ExactMath, LOBES>(c, x, io, q, n);
 }
 
 // Derivative of 1/k - coth(k): reference polynomial below 1 (distribution.h:

@@ -82,48 +82,8 @@ __device__ __forceinline__ void mlp_forward(const DevCfg& c, const float* __rest
 template <int K, int P, int NOUT>
 __device__ __forceinline__ void mixture_query(const DevCfg& c, const float* out, const QueryIO& io, uint32_t q,
                                               uint32_t n) {
-    if constexpr (P == 6) {
-        asg_query<K>([&](int j) { return out[j]; }, io, q, n);
-        return;
-    }
-    Mixture<K> m;
-    build_mixture<K>(c, [&](int j) { return out[j]; }, m);
-    float th[K] = {}, ph[K] = {};
-    if (io.normal) {
-        const V3 nrm = mk(io.normal[q], io.normal[(size_t)n + q], io.normal[2 * (size_t)n + q]);
-        apply_cosine<K>(c, nrm, m, io.lobes_out != nullptr, th, ph);
-    } else if (io.lobes_out) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const bool r = c.mapping == MAP_REFERENCE;
-            th[i] = kPi * (1.0f / (1.0f + fm::exp(-(r ? out[4 * i + 2] : out[4 * i]))));
-            ph[i] = kTwoPi * (1.0f / (1.0f + fm::exp(-(r ? out[4 * i + 3] : out[4 * i + 1]))));
-        }
-    }
-    if (io.lobes_out) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            io.lobes_out[(size_t)(5 * i + 0) * n + q] = m.lambda[i];
-            io.lobes_out[(size_t)(5 * i + 1) * n + q] = m.kappa[i];
-            io.lobes_out[(size_t)(5 * i + 2) * n + q] = th[i];
-            io.lobes_out[(size_t)(5 * i + 3) * n + q] = ph[i];
-            io.lobes_out[(size_t)(5 * i + 4) * n + q] = m.weight[i];
-        }
-    }
-    if (io.u) {
-        uint32_t lobe;
-        const V3 d = mixture_sample<K>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
-        io.out_dir[q] = d.x;
-        io.out_dir[(size_t)n + q] = d.y;
-        io.out_dir[2 * (size_t)n + q] = d.z;
-        io.out_pdf[q] = mixture_pdf<K>(c, m, d);
-        if (io.out_lobe) io.out_lobe[q] = lobe;
-    }
-    for (int e = 0; e < io.n_eval; ++e) {
-        const float* ed = io.eval_dirs + (size_t)e * 3 * n;
-        const V3 w = mk(ed[q], ed[(size_t)n + q], ed[2 * (size_t)n + q]);
-        io.out_eval_pdf[(size_t)e * n + q] = mixture_pdf<K>(c, m, w);
-    }
+    if constexpr (P == 6) asg_query<K>([&](int j) { return out[j]; }, io, q, n);
+    else vmf_query_cfg<K, true>(c, [&](int j) { return out[j]; }, io, q, n);
 }
 
 // Weights to LDS when they fit; returns the weights' base and this thread's activation column.

@@ -160,27 +160,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
         if constexpr (P == 6) {
             if (io.u || io.n_eval > 0) asg_query<K>([&](int jj) { return out[jj]; }, io, q, n);
         } else if (io.u || io.n_eval > 0) {
-            Mixture<K> m;
-            build_mixture<K>(c, [&](int jj) { return out[jj]; }, m);
-            float th[K], ph[K];
-            if (io.normal) {
-                const V3 nrm = mk(io.normal[q], io.normal[(size_t)n + q], io.normal[2 * (size_t)n + q]);
-                apply_cosine<K>(c, nrm, m, false, th, ph);
-            }
-            if (io.u) {
-                uint32_t lobe;
-                const V3 d = mixture_sample<K>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
-                io.out_dir[q] = d.x;
-                io.out_dir[(size_t)n + q] = d.y;
-                io.out_dir[2 * (size_t)n + q] = d.z;
-                io.out_pdf[q] = mixture_pdf<K>(c, m, d);
-                if (io.out_lobe) io.out_lobe[q] = lobe;
-            }
-            for (int e = 0; e < io.n_eval; ++e) {
-                const float* ed = io.eval_dirs + (size_t)e * 3 * n;
-                io.out_eval_pdf[(size_t)e * n + q] =
-                    mixture_pdf<K>(c, m, mk(ed[q], ed[(size_t)n + q], ed[2 * (size_t)n + q]));
-            }
+            vmf_query_cfg<K, false>(c, [&](int jj) { return out[jj]; }, io, q, n);
         }
     }
 }

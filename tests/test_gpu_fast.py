@@ -1,0 +1,112 @@
+"""The "fast" numerics: the mixture stage with hardware transcendentals.
+
+No bitwise oracle exists for the hardware v_exp/v_log/v_sin/v_cos/v_rcp
+results, so the FP32 path is checked against the oracle's FP64 mixture on the
+same raw outputs (pdfs within 2e-5 relative, sampled directions within 2e-5,
+lobe choices equal except where u0 lies within 1e-5 of a weight boundary);
+the fused MFMA path must still equal the consumer kernel bit for bit on its
+own raw outputs (same device code), and samples must follow the pdf (chi
+square), as for the exact functions.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(prec, mapping="softmax"):
+    import npmg
+
+    c = npmg.benchmark_config("small", precision=prec)
+    c["mixture"] = dict(c["mixture"], numerics="fast", mapping=mapping)
+    return c
+
+
+@pytest.mark.parametrize("mapping", ["softmax", "reference"])
+@pytest.mark.parametrize("cosine", [False, True])
+def test_fast_mixture_vs_fp64(gpu, oracle, mapping, cosine):
+    import torch
+    import npmg
+    import pyoracle
+
+    cfg = _cfg("fp32", mapping)
+    net = npmg.GuidingNetwork(cfg, seed=7)
+    n = 8192
+    q = npmg.synth_queries(5, n)
+    rng = np.random.default_rng(2)
+    raw = rng.normal(0, 1.5, (net.padded_output, n)).astype(np.float32)
+    nrm = q["normal"] if cosine else None
+    out = net.mixture_sample_pdf(torch.from_numpy(raw).cuda(), nrm, q["u"], q["eval_dirs"], cosine_aware=cosine)
+    h = {k: v.cpu().numpy() for k, v in q.items()}
+    oc = pyoracle.make_cfg(cfg)
+    gold = oracle.mixture(oc, raw.astype(np.float64), h["normal"] if cosine else None, h["u"], h["eval_dirs"],
+                          dtype=np.float64)
+    got = out["eval_pdf"].cpu().numpy()
+    normal_range = gold["eval_pdf"] > 1e-30  # far below, denormal results may flush
+    rel = np.abs(got - gold["eval_pdf"])[normal_range] / gold["eval_pdf"][normal_range]
+    print(f"{mapping} cosine={cosine}: eval pdf rel err max {rel.max():.2e} median {np.median(rel):.2e}")
+    assert rel.max() < 2e-5
+    assert np.abs(got - gold["eval_pdf"])[~normal_range].max(initial=0) < 1e-30
+    lobe = out["lobe"].cpu().numpy().astype(np.uint32)
+    same = lobe == gold["lobe"]
+    assert same.mean() > 0.999
+    ok = same
+    d = np.abs(out["dir"].cpu().numpy() - gold["dir"]).max(axis=0)
+    assert np.quantile(d[ok], 0.999) < 2e-5 and d[ok].max() < 1e-3, (np.quantile(d[ok], 0.999), d[ok].max())
+    prel = np.abs(out["pdf"].cpu().numpy() - gold["pdf"]) / gold["pdf"]
+    assert np.quantile(prel[ok], 0.999) < 1e-4
+    net.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_fast_fused_equals_consumer(gpu, prec):
+    import npmg
+
+    net = npmg.GuidingNetwork(_cfg(prec), seed=7)
+    q = npmg.synth_queries(9, 8192)
+    out = net.sample_pdf(q["inputs"], q["normal"], q["u"], q["eval_dirs"], raw=True)
+    c = net.mixture_sample_pdf(out["raw"], q["normal"], q["u"], q["eval_dirs"])
+    for k in ("dir", "pdf", "eval_pdf", "lobe"):
+        assert np.array_equal(out[k].cpu().numpy().view(np.uint32), c[k].cpu().numpy().view(np.uint32)), k
+    net.close()
+
+
+def test_fast_samples_follow_pdf(gpu):
+    import torch
+    import npmg
+    from scipy import stats
+
+    NZ, NP, SUB = 32, 64, 8
+    net = npmg.GuidingNetwork(_cfg("bf16"), seed=23)
+    t = npmg.synth_train(5, 1 << 14)
+    loss = torch.empty(1, device="cuda")
+    for _ in range(30):
+        net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], loss)
+    q = npmg.synth_queries(77, 1)
+    n = 1 << 20
+    inp = q["inputs"][:, :1].repeat(1, n).contiguous()
+    nrm = q["normal"][:, :1].repeat(1, n).contiguous()
+    u = torch.from_numpy(np.random.default_rng(1).random((3, n), dtype=np.float32)).cuda()
+    out = net.sample_pdf(inp, nrm, u, raw=True)
+    w = out["dir"].cpu().numpy().astype(np.float64)
+    iz = np.minimum((np.clip(w[2], -1, 1) + 1) / 2 * NZ, NZ - 1).astype(np.int64)
+    ip = np.minimum(np.mod(np.arctan2(w[1], w[0]), 2 * np.pi) / (2 * np.pi) * NP, NP - 1).astype(np.int64)
+    counts = np.bincount(iz * NP + ip, minlength=NZ * NP)
+    z = -1 + (np.arange(NZ * SUB) + 0.5) * (2.0 / (NZ * SUB))
+    ph = (np.arange(NP * SUB) + 0.5) * (2 * np.pi / (NP * SUB))
+    Z, PH = np.meshgrid(z, ph, indexing="ij")
+    r = np.sqrt(1 - Z ** 2)
+    g = np.stack([r * np.cos(PH), r * np.sin(PH), Z]).reshape(3, NZ, SUB, NP, SUB).transpose(0, 1, 3, 2, 4)
+    grid = torch.from_numpy(g.reshape(3, -1).astype(np.float32)).cuda()
+    ng = grid.shape[1]
+    pdf = net.mixture_sample_pdf(out["raw"][:, :1].repeat(1, ng).contiguous(), nrm[:, :1].repeat(1, ng).contiguous(),
+                                 None, grid.reshape(1, 3, ng).contiguous())["eval_pdf"][0].cpu().numpy()
+    prob = pdf.astype(np.float64).reshape(NZ * NP, SUB * SUB).sum(1) * (4 * np.pi / ng)
+    assert abs(prob.sum() - 1) < 2e-3
+    expected = prob / prob.sum() * n
+    keep = expected >= 5
+    chi2 = float((((counts - expected) ** 2 / expected)[keep]).sum())
+    p = float(stats.chi2.sf(chi2, int(keep.sum())))
+    print(f"fast bf16: chi2 {chi2:.0f} dof {int(keep.sum())} p {p:.3f}")
+    assert p > 1e-4
+    net.close()
