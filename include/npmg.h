@@ -111,6 +111,14 @@ npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float
  * No host synchronisation. */
 npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
                             const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream);
+/* The training step of a network with a learned selection probability
+ * ("mixture": {"n_lobes": 7, "learn_selection": true}; parameters.h:6-9):
+ * the loss and adjoint of compute_dL_doutput_with_selection (train.cu:126-172)
+ * need each record's BSDF pdf, tgt_bsdf_pdf [n]. use_graph: replay as the
+ * captured graph, as npmg_train_step_graph. */
+npmg_status npmg_train_step_selection(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                                      const float* tgt_pdf, const float* tgt_bsdf_pdf, uint32_t n, int use_graph,
+                                      float* loss_out, npmg_stream stream);
 /* npmg_train_step as one HIP graph: captured on the first call for these
  * buffers and this n (the workspace is sized before the capture), replayed by
  * every later call with the same arguments; other arguments capture anew.
@@ -135,7 +143,13 @@ typedef struct npmg_queue {
     float* out_eval_pdf;
     uint32_t* out_lobe;
     uint32_t capacity;
+    float* out_selection; /* [capacity] learned selection probability (mixture.learn_selection), or NULL */
 } npmg_queue;
+
+/* npmg_sample_pdf with the outputs described by one npmg_queue (row stride n,
+ * its capacity is ignored), including out_selection. */
+npmg_status npmg_sample_pdf_ex(npmg_ctx* ctx, const npmg_queue* q, uint32_t n, uint32_t n_eval, uint32_t flags,
+                               npmg_stream stream);
 
 /* npmg_sample_pdf over n_queues (<= 16) queues of uneven sizes in one
  * persistent launch: workgroups pull 64-query tiles from a device work
@@ -219,6 +233,7 @@ typedef struct npmg_path_records {
     const float* normal;     /* [D][2][P] shading normal */
     const float* roughness;  /* [D][P] */
     const uint32_t* depth;   /* [P] recorded vertices per pixel (curDepth) */
+    const float* bsdf_pdf;   /* [D][P] BSDF pdf of the scattered direction (learned selection), or NULL */
     uint32_t max_depth;      /* D (MAX_TRAIN_DEPTH, parameters.h:8) */
     uint32_t n_pixels;       /* P */
 } npmg_path_records;
@@ -235,11 +250,13 @@ typedef struct npmg_train_tile {
  * a NaN position/direction/target or a zero or NaN pdf; target radiance
  * (L / thp where thp > 1e-5, else 0) * misWeight * bsdfVal. Outputs with row
  * stride capacity: in [9], tgt_dir [2] (theta, phi), tgt_rgb [3], tgt_pdf;
+ * tgt_bsdf_pdf (may be NULL; needs records->bsdf_pdf) the BSDF pdf of the
+ * scattered direction (GuidedOutput::bsdfPdf, learned selection);
  * *count (device) = records kept; those past capacity are dropped, so train
  * on min(count, capacity). records, tile, scene_box: host pointers. */
 npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, const npmg_train_tile* tile,
                                const float* scene_box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
-                               float* tgt_pdf, uint32_t* count, npmg_stream stream);
+                               float* tgt_pdf, float* tgt_bsdf_pdf, uint32_t* count, npmg_stream stream);
 
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 
@@ -258,6 +275,17 @@ npmg_status npmg_forward(npmg_ctx* ctx, const float* in, float* raw_out, uint32_
 npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* tgt_dir, const float* tgt_rgb,
                                  const float* tgt_pdf, uint32_t n, float loss_scale, float* dout, float* loss,
                                  npmg_stream stream);
+/* The same with the learned selection probability (compute_dL_doutput_with_
+ * selection, train.cu:126-172): tgt_bsdf_pdf [n]; loss -f/P log P with
+ * P = s (p + 1e-5) + (1 - s) bsdfPdf. */
+npmg_status npmg_mixture_adjoint_selection(npmg_ctx* ctx, const float* raw, const float* tgt_dir,
+                                           const float* tgt_rgb, const float* tgt_pdf, const float* tgt_bsdf_pdf,
+                                           uint32_t n, float loss_scale, float* dout, float* loss,
+                                           npmg_stream stream);
+/* Selection probability s = logistic(raw[4 K]) of each query
+ * (MixedSGWithSelection::extractSelectionProbability, distribution.h:233-237;
+ * the integrator's bsdfSamplingFraction is 1 - s, integrator.cpp:172-176). */
+npmg_status npmg_selection_probability(npmg_ctx* ctx, const float* raw, float* out, uint32_t n, npmg_stream stream);
 /* Backward of the network with the training weights given dL/draw [n_out][n]:
  * writes the gradient of every parameter (tcnn order) to dparams [n_params]
  * and, if denc is not NULL, dL/denc [encoded_width][n]. */

@@ -193,7 +193,7 @@ struct npmg_ctx {
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    const void* graph_key[6] = {};
+    const void* graph_key[7] = {};
     static constexpr int kPipe = 8;
     // launch shape of the query path (npmg_autotune): pipeline chunks, workgroups per CU (0: all that fit)
     uint32_t tune_chunks = 4;
@@ -285,9 +285,9 @@ struct npmg_ctx {
     }
 
     void check_fused() const {
-        if (cfg.precision == PREC_FP8) throw std::invalid_argument("precision fp8 is not available in this build");
         if (cfg.d.mixture == MIX_ASG && cfg.d.n_hidden < 1) throw std::invalid_argument("invalid network");
-        if (cfg.d.K != 8 && cfg.d.K != 16) throw std::invalid_argument("the fused kernels support 8 or 16 lobes");
+        if (cfg.d.K != 7 && cfg.d.K != 8 && cfg.d.K != 16)
+            throw std::invalid_argument("the fused kernels support 7, 8 or 16 lobes");
     }
 
     void check_train() const {
@@ -412,6 +412,7 @@ struct npmg_ctx {
             io.out_pdf = q.out_pdf;
             io.out_eval_pdf = q.out_eval_pdf;
             io.out_lobe = q.out_lobe;
+            io.out_sel = q.out_selection;
             qs.stride[i] = q.capacity;
             total_cap += q.capacity;
         }
@@ -538,7 +539,7 @@ struct npmg_ctx {
                                        (size_t)m * sizeof(float), (size_t)d.n_in, hipMemcpyDeviceToDevice, s),
                       "copy calibration queries");
             hip_check(launch_train_forward_fp32(d, wt_infer, infer_params() + cfg.n_mlp, tin, nullptr, nullptr,
-                                                nullptr, LossScale{}, t, m, n_cu, s),
+                                                nullptr, nullptr, LossScale{}, t, m, n_cu, s),
                       "calibration forward");
             hip_check(launch_fp8_amax(d, t.enc, t.acts, m, fp8_amax, s), "calibration maxima");
             hip_check(launch_pack_frags_fp8(d, infer_params(), frag_infer, fp8_amax, fp8_scales, s), "pack");
@@ -614,15 +615,17 @@ struct npmg_ctx {
 
     // Forward with the training weights keeping what the backward pass needs;
     // with targets, the loss and dL/draw (scaled by ls) go to ws.loss / ws.dout.
-    void train_forward(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, LossScale ls,
-                       hipStream_t s) {
+    // tb: BSDF pdfs of the records (learned selection only).
+    void train_forward(const float* in, const float* td, const float* tr, const float* tp, const float* tb, uint32_t n,
+                       LossScale ls, hipStream_t s) {
         ensure_ws(n);
         if (half()) {
             hip_check(launch_train_forward_mfma(cfg.d, cfg.precision, frag_train, htab_train, genc, in, mws, n, n_cu, s),
                       "train forward (mfma)");
-            if (td) hip_check(launch_adjoint(cfg.d, mws.raw, td, tr, tp, ls, ws.dout, ws.loss, n, s), "adjoint");
+            if (td) hip_check(launch_adjoint(cfg.d, mws.raw, td, tr, tp, tb, ls, ws.dout, ws.loss, n, s), "adjoint");
         } else {
-            hip_check(launch_train_forward_fp32(cfg.d, wt_train, master + cfg.n_mlp, in, td, tr, tp, ls, ws, n, n_cu, s),
+            hip_check(launch_train_forward_fp32(cfg.d, wt_train, master + cfg.n_mlp, in, td, tr, tp, tb, ls, ws, n, n_cu,
+                                                s),
                       "train forward");
         }
     }
@@ -660,9 +663,9 @@ struct npmg_ctx {
     // One optimisation step on a batch: forward + adjoint, backward, Adam/EMA,
     // loss mean. Enqueues kernels only (no allocation once ensure_ws(n) ran, no
     // host synchronisation, no memset nodes), so it can be captured.
-    void train_step(const float* in, const float* td, const float* tr, const float* tp, uint32_t n, float* loss_out,
-                    hipStream_t s) {
-        train_forward(in, td, tr, tp, n, state_scale(n), s);
+    void train_step(const float* in, const float* td, const float* tr, const float* tp, const float* tb, uint32_t n,
+                    float* loss_out, hipStream_t s) {
+        train_forward(in, td, tr, tp, tb, n, state_scale(n), s);
         train_backward(in, ws.dout, grad, nullptr, n, cfg.dynamic_scale, s);
         optimizer_step(grad, 0.0f, s);
         if (loss_out) hip_check(launch_mean(ws.loss, n, mean_partials, loss_out, s), "loss mean");
@@ -673,7 +676,7 @@ struct npmg_ctx {
         if (graph) (void)hipGraphDestroy(graph);
         graph_exec = nullptr;
         graph = nullptr;
-        std::fill(graph_key, graph_key + 6, nullptr);
+        std::fill(graph_key, graph_key + 7, nullptr);
     }
 
     // Adam + EMA. ls > 0: the gradient is divided by ls (split-pass API);
@@ -782,7 +785,7 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
     const npmg_status st = guarded(ctx, [&] {
         ctx->cfg = parse_config(json_config);
         const int K = ctx->cfg.d.K;
-        need(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, "n_lobes must be 1, 2, 4, 8 or 16");
+        need(K == 1 || K == 2 || K == 4 || K == 7 || K == 8 || K == 16, "n_lobes must be 1, 2, 4, 7, 8 or 16");
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
@@ -953,6 +956,33 @@ npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal,
         io.out_eval_pdf = out_eval_pdf;
         io.out_lobe = out_lobe;
         io.raw_out = raw_out;
+        ctx->query(io, n, true, S(stream));
+    });
+}
+
+npmg_status npmg_sample_pdf_ex(npmg_ctx* ctx, const npmg_queue* q, uint32_t n, uint32_t n_eval, uint32_t flags,
+                               npmg_stream stream) {
+    if (!ctx || !q) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n == 0 || q->in, "null input");
+        need(!q->u || (q->out_dir && q->out_pdf), "sampling needs out_dir and out_pdf");
+        need(n_eval == 0 || (q->eval_dirs && q->out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
+        need(!(flags & NPMG_COSINE_AWARE) || q->normal, "NPMG_COSINE_AWARE needs normal");
+        need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
+             "the anisotropic mixture has no cosine product");
+        need(!q->out_selection || ctx->cfg.d.sel, "out_selection needs mixture.learn_selection");
+        QueryIO io{};
+        io.in = q->in;
+        io.normal = (flags & NPMG_COSINE_AWARE) ? q->normal : nullptr;
+        io.u = q->u;
+        io.eval_dirs = q->eval_dirs;
+        io.n_eval = (int)n_eval;
+        io.out_dir = q->out_dir;
+        io.out_pdf = q->out_pdf;
+        io.out_eval_pdf = q->out_eval_pdf;
+        io.out_lobe = q->out_lobe;
+        io.out_sel = q->out_selection;
         ctx->query(io, n, true, S(stream));
     });
 }
@@ -1185,7 +1215,7 @@ npmg_status npmg_pack_queries(const float* pos, const float* wo, const float* no
 
 npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, const npmg_train_tile* tile,
                                const float* scene_box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
-                               float* tgt_pdf, uint32_t* count, npmg_stream stream) {
+                               float* tgt_pdf, float* tgt_bsdf_pdf, uint32_t* count, npmg_stream stream) {
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
         need(records && tile && scene_box && count, "null argument");
@@ -1200,8 +1230,9 @@ npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, 
         need(n_threads == 0 || (r.L && r.thp && r.pos && r.dir && r.wi_pdf && r.mis_weight && r.delta &&
                                 r.bsdf_val && r.wo && r.normal && r.roughness && r.depth),
              "null record field");
+        need(!tgt_bsdf_pdf || r.bsdf_pdf, "tgt_bsdf_pdf needs records->bsdf_pdf");
         PathRecords pr{r.L, r.thp, r.pos, r.dir, r.wi_pdf, r.mis_weight, r.delta, r.bsdf_val, r.wo, r.normal,
-                       r.roughness, r.depth, r.max_depth, r.n_pixels};
+                       r.roughness, r.depth, r.bsdf_pdf, r.max_depth, r.n_pixels};
         TrainTile tt{};
         for (int a = 0; a < 2; ++a) {
             tt.offset[a] = tile->offset[a];
@@ -1222,7 +1253,7 @@ npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, 
             ctx->pack_scratch = dalloc<uint32_t>(need_words);
             ctx->pack_cap = need_words;
         }
-        hip_check(launch_pack_training(pr, tt, box, capacity, in, tgt_dir, tgt_rgb, tgt_pdf, count,
+        hip_check(launch_pack_training(pr, tt, box, capacity, in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf_pdf, count,
                                        ctx->pack_scratch, S(stream)),
                   "pack training");
     });
@@ -1267,7 +1298,7 @@ npmg_status npmg_mixture_sample_pdf(npmg_ctx* ctx, const float* raw, const float
         need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
         need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
              "the anisotropic mixture has no cosine product");
-        need(ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the mixture kernel supports 8 or 16 lobes");
+        need(ctx->cfg.d.K == 7 || ctx->cfg.d.K == 8 || ctx->cfg.d.K == 16, "the mixture kernel supports 7, 8 or 16 lobes");
         need(!lobes_out || ctx->cfg.d.mixture != MIX_ASG, "lobe parameters are written for vMF lobes only");
         QueryIO io{};
         io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
@@ -1309,9 +1340,35 @@ npmg_status npmg_mixture_adjoint(npmg_ctx* ctx, const float* raw, const float* t
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
         need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && dout && loss), "null buffer");
-        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf,
+        need(!ctx->cfg.d.sel, "learned selection: use npmg_mixture_adjoint_selection");
+        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf, nullptr,
                                  LossScale{nullptr, loss_scale, n ? 1.0f / (float)n : 0.0f}, dout, loss, n, S(stream)),
                   "adjoint");
+    });
+}
+
+npmg_status npmg_mixture_adjoint_selection(npmg_ctx* ctx, const float* raw, const float* tgt_dir,
+                                           const float* tgt_rgb, const float* tgt_pdf, const float* tgt_bsdf_pdf,
+                                           uint32_t n, float loss_scale, float* dout, float* loss,
+                                           npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(ctx->cfg.d.sel, "the configuration has no learned selection (mixture.learn_selection)");
+        need(n == 0 || (raw && tgt_dir && tgt_rgb && tgt_pdf && tgt_bsdf_pdf && dout && loss), "null buffer");
+        hip_check(launch_adjoint(ctx->cfg.d, raw, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf_pdf,
+                                 LossScale{nullptr, loss_scale, n ? 1.0f / (float)n : 0.0f}, dout, loss, n, S(stream)),
+                  "adjoint");
+    });
+}
+
+npmg_status npmg_selection_probability(npmg_ctx* ctx, const float* raw, float* out, uint32_t n, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(ctx->cfg.d.sel, "the configuration has no learned selection (mixture.learn_selection)");
+        need(n == 0 || (raw && out), "null buffer");
+        QueryIO io{};
+        io.out_sel = out;
+        hip_check(launch_mixture(ctx->cfg.d, raw, io, n, S(stream)), "selection");
     });
 }
 
@@ -1321,7 +1378,8 @@ npmg_status npmg_backward(npmg_ctx* ctx, const float* in, const float* dout, uin
     return guarded(ctx, [&] {
         ctx->check_train();
         need(n > 0 && in && dout && dparams, "null buffer or empty batch");
-        ctx->train_forward(in, nullptr, nullptr, nullptr, n, LossScale{nullptr, 1.0f, 1.0f / (float)n}, S(stream));
+        ctx->train_forward(in, nullptr, nullptr, nullptr, nullptr, n, LossScale{nullptr, 1.0f, 1.0f / (float)n},
+                           S(stream));
         ctx->train_backward(in, dout, dparams, denc, n, false, S(stream));
     });
 }
@@ -1335,14 +1393,64 @@ npmg_status npmg_optimizer_step(npmg_ctx* ctx, const float* dparams, float loss_
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// One train step, eager or as the captured graph (replayed while the buffers
+// and n are those it was captured for).
+void train_step_entry(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
+                      const float* tgt_bsdf, uint32_t n, float* loss_out, bool use_graph, hipStream_t stream) {
+    ctx->check_train();
+    need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
+    need(!ctx->cfg.d.sel || tgt_bsdf, "learned selection: train with npmg_train_step_selection (BSDF pdfs)");
+    if (!use_graph) {
+        ctx->timed(NPMG_TIME_TRAIN_STEP, stream,
+                   [&] { ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, n, loss_out, stream); });
+        return;
+    }
+    const void* key[7] = {in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, loss_out, (const void*)(uintptr_t)n};
+    if (!ctx->graph_exec || !std::equal(key, key + 7, ctx->graph_key)) {
+        ctx->drop_graph();
+        ctx->ensure_ws(n);  // every allocation happens before the capture
+        hip_check(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeRelaxed), "begin capture");
+        hipGraph_t g = nullptr;
+        try {
+            ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, n, loss_out, ctx->capture);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->capture, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(ctx->capture, &g), "end capture");
+        ctx->graph = g;
+        hip_check(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
+        std::copy(key, key + 7, ctx->graph_key);
+    }
+    ctx->timed(NPMG_TIME_TRAIN_STEP, stream,
+               [&] { hip_check(hipGraphLaunch(ctx->graph_exec, stream), "launch graph"); });
+}
+
+}  // namespace
+
+extern "C" {
+
 npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
                             const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream) {
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
-        ctx->check_train();
-        need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
-        ctx->timed(NPMG_TIME_TRAIN_STEP, S(stream),
-                   [&] { ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, S(stream)); });
+        train_step_entry(ctx, in, tgt_dir, tgt_rgb, tgt_pdf, nullptr, n, loss_out, false, S(stream));
+    });
+}
+
+npmg_status npmg_train_step_selection(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                                      const float* tgt_pdf, const float* tgt_bsdf_pdf, uint32_t n, int use_graph,
+                                      float* loss_out, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(ctx->cfg.d.sel, "the configuration has no learned selection (mixture.learn_selection)");
+        need(tgt_bsdf_pdf != nullptr, "null BSDF pdfs");
+        train_step_entry(ctx, in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf_pdf, n, loss_out, use_graph != 0, S(stream));
     });
 }
 
@@ -1368,28 +1476,7 @@ npmg_status npmg_train_step_graph(npmg_ctx* ctx, const float* in, const float* t
                                   const float* tgt_pdf, uint32_t n, float* loss_out, npmg_stream stream) {
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
-        ctx->check_train();
-        need(n > 0 && in && tgt_dir && tgt_rgb && tgt_pdf, "null buffer or empty batch");
-        const void* key[6] = {in, tgt_dir, tgt_rgb, tgt_pdf, loss_out, (const void*)(uintptr_t)n};
-        if (!ctx->graph_exec || !std::equal(key, key + 6, ctx->graph_key)) {
-            ctx->drop_graph();
-            ctx->ensure_ws(n);  // every allocation happens before the capture
-            hip_check(hipStreamBeginCapture(ctx->capture, hipStreamCaptureModeRelaxed), "begin capture");
-            hipGraph_t g = nullptr;
-            try {
-                ctx->train_step(in, tgt_dir, tgt_rgb, tgt_pdf, n, loss_out, ctx->capture);
-            } catch (...) {
-                (void)hipStreamEndCapture(ctx->capture, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            hip_check(hipStreamEndCapture(ctx->capture, &g), "end capture");
-            ctx->graph = g;
-            hip_check(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
-            std::copy(key, key + 6, ctx->graph_key);
-        }
-        ctx->timed(NPMG_TIME_TRAIN_STEP, S(stream),
-                   [&] { hip_check(hipGraphLaunch(ctx->graph_exec, S(stream)), "launch graph"); });
+        train_step_entry(ctx, in, tgt_dir, tgt_rgb, tgt_pdf, nullptr, n, loss_out, true, S(stream));
     });
 }
 

@@ -355,9 +355,10 @@ __device__ __forceinline__ float asg_adjoint(const DevCfg& c, Raw&& x, float th_
 // The loss adjoint of either mixture (P raw outputs per lobe).
 template <int K, int P, class Raw, class Out>
 __device__ __forceinline__ float lobe_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
-                                              float b, float wipdf, float ls_over_n, float clamp_hi, Out&& out) {
+                                              float b, float wipdf, float bsdf_pdf, float ls_over_n, float clamp_hi,
+                                              Out&& out) {
     if constexpr (P == 6) return asg_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, ls_over_n, clamp_hi, out);
-    else return mixture_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, ls_over_n, clamp_hi, out);
+    else return mixture_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, bsdf_pdf, ls_over_n, clamp_hi, out);
 }
 
 }  // namespace dev

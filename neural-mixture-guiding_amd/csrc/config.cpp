@@ -168,7 +168,12 @@ Config parse_config(const std::string& text) {
     else if (ieq(num, "fast")) d.numerics = NUM_FAST;
     else bad("mixture numerics must be reference, stable or fast");
     d.cos_kappa = mix.kind == Json::Object ? (float)mix.number("cosine_kappa", 2.18853) : 2.18853f;
-    d.n_out = d.lobe_params * d.K;
+    // learned selection (distribution.h:204-241, parameters.h:6-9): one more
+    // output, the logit of the guiding distribution's selection probability;
+    // with 7 lobes it takes output 29 of 32, as in the reference
+    d.sel = mix.kind == Json::Object && mix.boolean("learn_selection", false) ? 1 : 0;
+    if (d.sel && (d.mixture != MIX_VMF || d.K != 7)) bad("learn_selection needs the vMF mixture with 7 lobes");
+    d.n_out = d.lobe_params * d.K + d.sel;
     d.n_out_pad = (d.n_out + 15) / 16 * 16;
 
     // ---- network (tiny-cuda-nn networks/fully_fused_mlp.h) ----

@@ -40,6 +40,7 @@ struct DevCfg {
     uint32_t layer_off[kMaxLayers];  // offset of each weight matrix in the parameter vector
     int K, mapping, numerics, mixture;
     int lobe_params;  // raw outputs per lobe: 4 (vMF), 6 (anisotropic)
+    int sel;          // 1: learned selection probability in output lobe_params * K (MixedSGWithSelection)
     float cos_kappa, grad_clamp;
     float clamp_scale;  // loss scale at which grad_clamp applies (threshold grad_clamp * scale / clamp_scale)
 };

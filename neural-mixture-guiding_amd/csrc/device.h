@@ -553,6 +553,7 @@ __device__ __forceinline__ void vmf_query(const DevCfg& c, Raw&& x, const QueryI
             io.lobes_out[(size_t)(5 * i + 4) * n + q] = m.weight[i];
         }
     }
+    if (c.sel && io.out_sel) io.out_sel[q] = MT::div(1.0f, 1.0f + MT::exp(-x(4 * K)));  // extractSelectionProbability
     if (io.u) {
         uint32_t lobe;
         const V3 d = mixture_sample<K, MT>(c, m, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
@@ -596,10 +597,15 @@ __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
 // dp/dlambda_s is evaluated as (v_s - p) / Lambda, the closed form of the
 // reference's O(K^2) expression. `ls_over_n` is loss_scale / batch size;
 // clamp_hi the upper clamp of the scaled gradient (0: none).
+// With a learned selection probability s = logistic(x[4K]) (c.sel), the loss
+// and gradients of compute_dL_doutput_with_selection (train.cu:126-172,
+// distribution.h:216-231): P = s (p + eps) + (1 - s) bsdf_pdf, loss
+// -f/P log P, dL/dx = -f/P^2 dP/dx with dP/dlobe = s dp/dlobe and the
+// selection slot 0.01 (p - bsdf_pdf) s (1 - s); no NaN zeroing or clamp.
 template <int K, class Raw, class Out>
 __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
-                                                 float b, float wipdf, float ls_over_n, float clamp_hi,
-                                                 Out&& out) {
+                                                 float b, float wipdf, float bsdf_pdf, float ls_over_n,
+                                                 float clamp_hi, Out&& out) {
     const bool ref = c.numerics == NUM_REFERENCE;
     Mixture<K> m;
     build_mixture<K>(c, x, m);
@@ -614,11 +620,25 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
         p = fm::fma_(m.weight[s], vmf[s], p);
     }
     const float guide = p + kEps;
-    const float prefix = -Li / wiPdf / guide * ls_over_n;
-    const float loss = -Li / wiPdf * fm::log(guide);
+    float prefix, loss, sel_s = 1.0f, sel_grad = 0.0f;
+    if (c.sel) {
+        const float s = 1.0f / (1.0f + fm::exp(-x(4 * K)));
+        const float P = s * guide + (1.0f - s) * bsdf_pdf;
+        prefix = -Li / P / P * ls_over_n;
+        loss = -Li / P * fm::log(P);
+        sel_s = s;
+        sel_grad = 0.01f * prefix * (p - bsdf_pdf) * (s * (1.0f - s));
+    } else {
+        prefix = -Li / wiPdf / guide * ls_over_n;
+        loss = -Li / wiPdf * fm::log(guide);
+    }
+    // dp/d(param) of a lobe, times s with a learned selection
+    auto sc = [&](float v) { return c.sel ? v * sel_s : v; };
     auto finish = [&](int j, float v) {
-        if (!fm::isfinite_(v)) v = 0.0f;
-        if (clamp_hi > 0.0f) v = smin(clamp_hi, v);
+        if (!c.sel) {
+            if (!fm::isfinite_(v)) v = 0.0f;
+            if (clamp_hi > 0.0f) v = smin(clamp_hi, v);
+        }
         out(j, v);
     };
 #pragma unroll
@@ -644,24 +664,25 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
             const float d1 = (!ref && (x1 < -10.0f || x1 > 15.0f)) ? 0.0f : e1;
             const float s2 = 1.0f / (1.0f + fm::exp(-x2));
             const float s3 = 1.0f / (1.0f + fm::exp(-x3));
-            finish(4 * s + 0, prefix * dl * d0);
-            finish(4 * s + 1, prefix * dk * d1);
-            finish(4 * s + 2, prefix * dth * kPi * (s2 * (1.0f - s2)));
-            finish(4 * s + 3, prefix * dph * kTwoPi * (s3 * (1.0f - s3)));
+            finish(4 * s + 0, prefix * sc(dl) * d0);
+            finish(4 * s + 1, prefix * sc(dk) * d1);
+            finish(4 * s + 2, prefix * sc(dth) * kPi * (s2 * (1.0f - s2)));
+            finish(4 * s + 3, prefix * sc(dph) * kTwoPi * (s3 * (1.0f - s3)));
         } else {
             const float x0 = x(4 * s + 0), x1 = x(4 * s + 1), x2 = x(4 * s + 2);
             const float s0 = 1.0f / (1.0f + fm::exp(-x0));
             const float s1 = 1.0f / (1.0f + fm::exp(-x1));
             const float ek = fm::exp(x2);
             const float dkx = (ek > 1e-2f && ek < 1e3f) ? ek : 0.0f;
-            finish(4 * s + 0, prefix * dth * kPi * (s0 * (1.0f - s0)));
-            finish(4 * s + 1, prefix * dph * kTwoPi * (s1 * (1.0f - s1)));
-            finish(4 * s + 2, prefix * dk * dkx);
+            finish(4 * s + 0, prefix * sc(dth) * kPi * (s0 * (1.0f - s0)));
+            finish(4 * s + 1, prefix * sc(dph) * kTwoPi * (s1 * (1.0f - s1)));
+            finish(4 * s + 2, prefix * sc(dk) * dkx);
             // lambda_k = exp(logit_k - max): d lambda_k / d logit_s = delta_ks lambda_s.
-            finish(4 * s + 3, prefix * dl * m.lambda[s]);
+            finish(4 * s + 3, prefix * sc(dl) * m.lambda[s]);
         }
     }
-    for (int j = 4 * K; j < c.n_out_pad; ++j) out(j, 0.0f);
+    if (c.sel) out(4 * K, sel_grad);
+    for (int j = 4 * K + c.sel; j < c.n_out_pad; ++j) out(j, 0.0f);
     return loss;
 }
 

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) k_pack_queries(const float* __restrict__ 
 }
 
 struct TrainRecord {
-    float in[9], dir[2], rgb[3], pdf;
+    float in[9], dir[2], rgb[3], pdf, bsdf_pdf;
 };
 
 // One recorded vertex -> one training record, and whether the reference keeps it.
@@ -77,6 +77,7 @@ __device__ __forceinline__ bool train_record(const PathRecords& r, const SceneBo
     t.dir[0] = r.dir[s2];
     t.dir[1] = r.dir[s2 + P];
     t.pdf = r.wi_pdf[s1];
+    t.bsdf_pdf = r.bsdf_pdf ? r.bsdf_pdf[s1] : 0.0f;
     const bool bad = __builtin_isnan(t.in[0]) || __builtin_isnan(t.in[1]) || __builtin_isnan(t.in[2]) ||
                      __builtin_isnan(t.dir[0]) || __builtin_isnan(t.dir[1]) || __builtin_isnan(t.in[3]) ||
                      __builtin_isnan(t.in[4]) || __builtin_isnan(t.in[5]) || __builtin_isnan(t.pdf) ||
@@ -151,7 +152,7 @@ __global__ void __launch_bounds__(256) k_pack_train_write(const PathRecords r, c
                                                           uint32_t n_threads, const uint32_t* __restrict__ block_base,
                                                           uint32_t capacity, float* __restrict__ in,
                                                           float* __restrict__ tgt_dir, float* __restrict__ tgt_rgb,
-                                                          float* __restrict__ tgt_pdf) {
+                                                          float* __restrict__ tgt_pdf, float* __restrict__ tgt_bsdf) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     uint32_t total;
     uint32_t idx = block_base[blockIdx.x] + block_exclusive_scan(pixel_count(r, tt, box, t, n_threads), &total);
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(256) k_pack_train_write(const PathRecords r, c
 #pragma unroll
             for (int k = 0; k < 3; ++k) tgt_rgb[(size_t)k * C + idx] = rec.rgb[k];
             tgt_pdf[idx] = rec.pdf;
+            if (tgt_bsdf) tgt_bsdf[idx] = rec.bsdf_pdf;
         }
         ++idx;
     }
@@ -191,8 +193,8 @@ size_t pack_training_scratch(const TrainTile& t) {
 }
 
 hipError_t launch_pack_training(const PathRecords& r, const TrainTile& t, const SceneBox& box, uint32_t capacity,
-                                float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf, uint32_t* count,
-                                uint32_t* scratch, hipStream_t s) {
+                                float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf, float* tgt_bsdf,
+                                uint32_t* count, uint32_t* scratch, hipStream_t s) {
     const uint64_t nt = (uint64_t)t.res[0] * t.res[1];
     if (nt >= (1ull << 32)) return hipErrorInvalidValue;
     const uint32_t n_threads = (uint32_t)nt, n_blocks = (n_threads + 255u) / 256u;
@@ -201,7 +203,7 @@ hipError_t launch_pack_training(const PathRecords& r, const TrainTile& t, const 
     hipLaunchKernelGGL(k_pack_train_scan, dim3(1), dim3(256), 0, s, scratch, n_blocks, count);
     if (n_blocks)
         hipLaunchKernelGGL(k_pack_train_write, dim3(n_blocks), dim3(256), 0, s, r, t, box, n_threads, scratch,
-                           capacity, in, tgt_dir, tgt_rgb, tgt_pdf);
+                           capacity, in, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf);
     return hipGetLastError();
 }
 

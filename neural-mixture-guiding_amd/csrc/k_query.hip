@@ -117,7 +117,7 @@ __device__ __forceinline__ void query_one_fp32(const DevCfg& c, const float* sw,
 #pragma unroll
         for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
     }
-    if (io.u || io.n_eval > 0) mixture_query<K, P, NOUT>(c, out, io, q, n);
+    if (io.u || io.n_eval > 0 || io.out_sel) mixture_query<K, P, NOUT>(c, out, io, q, n);
 }
 
 template <int W, int K, int P, bool LDS_W>
@@ -239,8 +239,10 @@ hipError_t launch_query_fp32(const DevCfg& c, const float* wt, const float* tabl
     if (c.width == WW && c.K == KK && c.lobe_params == PP)                           \
         return lds ? launch_query_t<WW, KK, PP, true>(c, wt, table, io, n, n_cu, s) \
                    : launch_query_t<WW, KK, PP, false>(c, wt, table, io, n, n_cu, s);
+    NPMG_Q(64, 7, 4)
     NPMG_Q(64, 8, 4)
     NPMG_Q(64, 16, 4)
+    NPMG_Q(128, 7, 4)
     NPMG_Q(128, 8, 4)
     NPMG_Q(128, 16, 4)
     NPMG_Q(64, 16, 6)
@@ -272,8 +274,10 @@ hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const floa
     if (c.width == WW && c.K == KK && c.lobe_params == PP)                   \
         return lds ? launch_queues_t<WW, KK, PP, true>(c, wt, table, qs, n_cu, s) \
                    : launch_queues_t<WW, KK, PP, false>(c, wt, table, qs, n_cu, s);
+    NPMG_QF(64, 7, 4)
     NPMG_QF(64, 8, 4)
     NPMG_QF(64, 16, 4)
+    NPMG_QF(128, 7, 4)
     NPMG_QF(128, 8, 4)
     NPMG_QF(128, 16, 4)
     NPMG_QF(64, 16, 6)
@@ -286,8 +290,10 @@ hipError_t launch_mixture(const DevCfg& c, const float* raw, const QueryIO& io, 
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256);
     if (c.lobe_params == 6) hipLaunchKernelGGL((k_mixture<16, 6>), grid, dim3(256), 0, s, c, raw, io, n);
-    else if (c.K <= 8) hipLaunchKernelGGL((k_mixture<8, 4>), grid, dim3(256), 0, s, c, raw, io, n);
-    else hipLaunchKernelGGL((k_mixture<16, 4>), grid, dim3(256), 0, s, c, raw, io, n);
+    else if (c.K == 7) hipLaunchKernelGGL((k_mixture<7, 4>), grid, dim3(256), 0, s, c, raw, io, n);
+    else if (c.K == 8) hipLaunchKernelGGL((k_mixture<8, 4>), grid, dim3(256), 0, s, c, raw, io, n);
+    else if (c.K == 16) hipLaunchKernelGGL((k_mixture<16, 4>), grid, dim3(256), 0, s, c, raw, io, n);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

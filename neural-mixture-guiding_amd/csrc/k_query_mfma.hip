@@ -224,8 +224,10 @@ hipError_t launch_mfma_queues_type(const DevCfg& c, const void* wfrag, const voi
 #define NPMG_QQ(WW, KK, PP)                                \
     if (c.width == WW && c.K == KK && c.lobe_params == PP) \
         return launch_mfma_queues_t<T, O, WW, KK, PP>(c, wfrag, table, qs, n_cu, bpc, qscale, s);
+    NPMG_QQ(64, 7, 4)
     NPMG_QQ(64, 8, 4)
     NPMG_QQ(64, 16, 4)
+    NPMG_QQ(128, 7, 4)
     NPMG_QQ(128, 8, 4)
     NPMG_QQ(128, 16, 4)
     NPMG_QQ(64, 16, 6)
@@ -241,8 +243,10 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
 #define NPMG_QM(WW, KK, PP)                                      \
     if (c.width == WW && c.K == KK && c.lobe_params == PP)       \
         return launch_mfma_t<T, O, WW, KK, PP>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, bpc, qscale, s);
+    NPMG_QM(64, 7, 4)
     NPMG_QM(64, 8, 4)
     NPMG_QM(64, 16, 4)
+    NPMG_QM(128, 7, 4)
     NPMG_QM(128, 8, 4)
     NPMG_QM(128, 16, 4)
     NPMG_QM(64, 16, 6)

@@ -25,8 +25,10 @@ namespace npmg {
                                                                      hipStream_t);
 
 #if defined(NPMG_INST_vmf)
+NPMG_INST(64, 7, 4)
 NPMG_INST(64, 8, 4)
 NPMG_INST(64, 16, 4)
+NPMG_INST(128, 7, 4)
 NPMG_INST(128, 8, 4)
 NPMG_INST(128, 16, 4)
 #elif defined(NPMG_INST_asg)

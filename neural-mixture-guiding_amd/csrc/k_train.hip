@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
                                                                const float* __restrict__ in,
                                                                const float* __restrict__ tgt_dir,
                                                                const float* __restrict__ tgt_rgb,
-                                                               const float* __restrict__ tgt_pdf, const LossScale ls,
+                                                               const float* __restrict__ tgt_pdf,
+                                                               const float* __restrict__ tgt_bsdf, const LossScale ls,
                                                                const TrainWS ws, uint32_t n) {
     constexpr int BS = block_size<W>();
     constexpr int NOUT = (P * K + 15) / 16 * 16;
@@ -93,8 +94,8 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
         if (tgt_dir) {
             const float loss = lobe_adjoint<K, P>(
                 c, [&](int jj) { return out[jj]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
-                [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
+                tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], tgt_bsdf ? tgt_bsdf[q] : 0.0f,
+                ls.over_n(), ls.clamp(c), [&](int jj, float v) { ws.dout[(size_t)jj * n + q] = v; });
             ws.loss[q] = loss;
         }
     }
@@ -104,14 +105,15 @@ __global__ void __launch_bounds__(block_size<W>()) k_train_fwd(const DevCfg c, c
 template <int K, int P>
 __global__ void __launch_bounds__(256) k_adjoint(const DevCfg c, const float* __restrict__ raw,
                                                  const float* __restrict__ tgt_dir, const float* __restrict__ tgt_rgb,
-                                                 const float* __restrict__ tgt_pdf, const LossScale ls,
-                                                 float* __restrict__ dout, float* __restrict__ loss, uint32_t n) {
+                                                 const float* __restrict__ tgt_pdf, const float* __restrict__ tgt_bsdf,
+                                                 const LossScale ls, float* __restrict__ dout,
+                                                 float* __restrict__ loss, uint32_t n) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     if (q >= n) return;
     loss[q] = lobe_adjoint<K, P>(
         c, [&](int j) { return raw[(size_t)j * n + q]; }, tgt_dir[q], tgt_dir[(size_t)n + q], tgt_rgb[q],
-        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], ls.over_n(), ls.clamp(c),
-        [&](int j, float v) { dout[(size_t)j * n + q] = v; });
+        tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q], tgt_pdf[q], tgt_bsdf ? tgt_bsdf[q] : 0.0f, ls.over_n(),
+        ls.clamp(c), [&](int j, float v) { dout[(size_t)j * n + q] = v; });
 }
 
 // Backward: g_{l-1} = (W_l^T g_l) masked by ReLU, from the output layer down,
@@ -353,8 +355,8 @@ __global__ void __launch_bounds__(256) k_mean_final(const float* __restrict__ pa
 
 template <int W, int K, int P, bool LDS_W>
 hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, const float* in, const float* td,
-                        const float* tr, const float* tp, LossScale ls, const TrainWS& ws, uint32_t n, int n_cu,
-                        hipStream_t s) {
+                        const float* tr, const float* tp, const float* tb, LossScale ls, const TrainWS& ws, uint32_t n,
+                        int n_cu, hipStream_t s) {
     constexpr int BS = block_size<W>();
     const uint32_t n_mlp = n_mlp_of(c);
     const size_t lds = (LDS_W ? ((n_mlp + 3u) & ~3u) * sizeof(float) : 0) + (size_t)W * BS * sizeof(float);
@@ -363,8 +365,8 @@ hipError_t launch_fwd_t(const DevCfg& c, const float* wt, const float* table, co
     if (e != hipSuccess) return e;
     const uint32_t tiles = (n + BS - 1) / BS;
     const uint32_t grid = tiles < (uint32_t)n_cu ? tiles : (uint32_t)n_cu;
-    hipLaunchKernelGGL((k_train_fwd<W, K, P, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, in, td, tr, tp, ls,
-                       ws, n);
+    hipLaunchKernelGGL((k_train_fwd<W, K, P, LDS_W>), dim3(grid), dim3(BS), lds, s, c, wt, table, in, td, tr, tp, tb,
+                       ls, ws, n);
     return hipGetLastError();
 }
 
@@ -389,16 +391,18 @@ hipError_t launch_bwd_t(const DevCfg& c, const float* w, const TrainWS& ws, floa
 bool fp32_weights_fit_lds(const DevCfg& c);
 
 hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
-                                     const float* td, const float* tr, const float* tp, LossScale ls, const TrainWS& ws,
-                                     uint32_t n, int n_cu, hipStream_t s) {
+                                     const float* td, const float* tr, const float* tp, const float* tb, LossScale ls,
+                                     const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const bool lds = fp32_weights_fit_lds(c);
 #define NPMG_F(WW, KK, PP)                                                                       \
     if (c.width == WW && c.K == KK && c.lobe_params == PP)                                       \
-        return lds ? launch_fwd_t<WW, KK, PP, WW == 64>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s) \
-                   : launch_fwd_t<WW, KK, PP, false>(c, wt, table, in, td, tr, tp, ls, ws, n, n_cu, s);
+        return lds ? launch_fwd_t<WW, KK, PP, WW == 64>(c, wt, table, in, td, tr, tp, tb, ls, ws, n, n_cu, s) \
+                   : launch_fwd_t<WW, KK, PP, false>(c, wt, table, in, td, tr, tp, tb, ls, ws, n, n_cu, s);
+    NPMG_F(64, 7, 4)
     NPMG_F(64, 8, 4)
     NPMG_F(64, 16, 4)
+    NPMG_F(128, 7, 4)
     NPMG_F(128, 8, 4)
     NPMG_F(128, 16, 4)
     NPMG_F(64, 16, 6)
@@ -487,12 +491,17 @@ hipError_t launch_mean(const float* x, uint32_t n, float* partials, float* out, 
 }
 
 hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* td, const float* tr, const float* tp,
-                          LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
+                          const float* tb, LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255) / 256);
-    if (c.lobe_params == 6) hipLaunchKernelGGL((k_adjoint<16, 6>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
-    else if (c.K == 8) hipLaunchKernelGGL((k_adjoint<8, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
-    else if (c.K == 16) hipLaunchKernelGGL((k_adjoint<16, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, ls, dout, loss, n);
+    if (c.lobe_params == 6)
+        hipLaunchKernelGGL((k_adjoint<16, 6>), grid, dim3(256), 0, s, c, raw, td, tr, tp, tb, ls, dout, loss, n);
+    else if (c.K == 7)
+        hipLaunchKernelGGL((k_adjoint<7, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, tb, ls, dout, loss, n);
+    else if (c.K == 8)
+        hipLaunchKernelGGL((k_adjoint<8, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, tb, ls, dout, loss, n);
+    else if (c.K == 16)
+        hipLaunchKernelGGL((k_adjoint<16, 4>), grid, dim3(256), 0, s, c, raw, td, tr, tp, tb, ls, dout, loss, n);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

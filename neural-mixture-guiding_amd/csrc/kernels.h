@@ -21,6 +21,7 @@ struct QueryIO {
     uint32_t* out_lobe;      // [n] or null
     float* raw_out;          // [n_out_pad][n] or null
     float* lobes_out;        // [K][5][n] or null (consumer kernel only)
+    float* out_sel;          // [n] learned selection probability, or null
 };
 
 // Several query streams (the per-bounce queues of a wavefront frame) served by
@@ -100,9 +101,12 @@ hipError_t launch_encode(const DevCfg& c, const float* table, const float* in, f
 hipError_t launch_grid_corners(const DevCfg& c, const float* in, uint32_t* idx, float* w, uint32_t n, hipStream_t s);
 
 // Training (FP32): forward storing activations (+ adjoint when tgt != null).
+// tgt_bsdf [n]: the BSDF sampling pdf of each record, for the learned
+// selection probability (null otherwise).
 hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const float* table, const float* in,
                                      const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
-                                     LossScale ls, const TrainWS& ws, uint32_t n, int n_cu, hipStream_t s);
+                                     const float* tgt_bsdf, LossScale ls, const TrainWS& ws, uint32_t n, int n_cu,
+                                     hipStream_t s);
 // Backward through the MLP: g of each hidden layer into ws.g, dL/d(grid
 // features) into dgrid [L F][n], dL/denc into denc (optional).
 hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
@@ -132,6 +136,7 @@ struct PathRecords {
     const float* normal;     // [D][2][P] shading normal (theta, phi)
     const float* roughness;  // [D][P]
     const uint32_t* depth;   // [P] recorded vertices (curDepth)
+    const float* bsdf_pdf;   // [D][P] BSDF pdf of the scattered direction, or null
     uint32_t max_depth, n_pixels;
 };
 // Training pixels: thread t -> pixel (t % res[0], t / res[0]) * stride + offset.
@@ -143,8 +148,8 @@ hipError_t launch_pack_queries(const float* pos, const float* wo, const float* n
 // Scratch: pack_training_scratch(tile) uint32 values.
 size_t pack_training_scratch(const TrainTile& t);
 hipError_t launch_pack_training(const PathRecords& r, const TrainTile& t, const SceneBox& box, uint32_t capacity,
-                                float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf, uint32_t* count,
-                                uint32_t* scratch, hipStream_t s);
+                                float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf, float* tgt_bsdf,
+                                uint32_t* count, uint32_t* scratch, hipStream_t s);
 
 // FP8 inference: fragments with per-layer weight scales and the fused kernel's
 // multipliers (fp8 = wscale [L], ascale [L], qscale [L + 1]); calibration maxima.
@@ -190,7 +195,8 @@ hipError_t launch_transpose_mlp(const DevCfg& c, const float* w, float* wt, hipS
 // Mean of n values into out[0] (fixed summation order). partials: 256 floats.
 hipError_t launch_mean(const float* x, uint32_t n, float* partials, float* out, hipStream_t s);
 hipError_t launch_adjoint(const DevCfg& c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
-                          const float* tgt_pdf, LossScale ls, float* dout, float* loss, uint32_t n, hipStream_t s);
+                          const float* tgt_pdf, const float* tgt_bsdf, LossScale ls, float* dout, float* loss,
+                          uint32_t n, hipStream_t s);
 
 hipError_t launch_synth_queries(uint64_t seed, uint64_t first, uint32_t n, float* in, float* normal, float* u,
                                 float* eval_dirs, hipStream_t s);

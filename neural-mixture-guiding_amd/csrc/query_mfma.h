@@ -159,7 +159,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
         }
         if constexpr (P == 6) {
             if (io.u || io.n_eval > 0) asg_query<K>([&](int jj) { return out[jj]; }, io, q, n);
-        } else if (io.u || io.n_eval > 0) {
+        } else if (io.u || io.n_eval > 0 || io.out_sel) {
             vmf_query_cfg<K, false>(c, [&](int jj) { return out[jj]; }, io, q, n);
         }
     }

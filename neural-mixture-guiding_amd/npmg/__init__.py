@@ -36,13 +36,15 @@ _sz = ctypes.c_size_t
 class NpmgQueue(ctypes.Structure):
     """npmg_queue of include/npmg.h: one queue's SoA buffers, row stride `capacity`."""
     _fields_ = [("in_", _vp), ("normal", _vp), ("u", _vp), ("eval_dirs", _vp), ("out_dir", _vp),
-                ("out_pdf", _vp), ("out_eval_pdf", _vp), ("out_lobe", _vp), ("capacity", _u32)]
+                ("out_pdf", _vp), ("out_eval_pdf", _vp), ("out_lobe", _vp), ("capacity", _u32),
+                ("out_selection", _vp)]
 
 
 class NpmgPathRecords(ctypes.Structure):
     """npmg_path_records of include/npmg.h (device pointers, SoA per depth)."""
-    _fields_ = [(k, _vp) for k in ("L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo",
-                                   "normal", "roughness", "depth")] + [("max_depth", _u32), ("n_pixels", _u32)]
+    FIELDS = ("L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo", "normal", "roughness",
+              "depth", "bsdf_pdf")
+    _fields_ = [(k, _vp) for k in FIELDS] + [("max_depth", _u32), ("n_pixels", _u32)]
 
 
 class NpmgTrainTile(ctypes.Structure):
@@ -85,7 +87,11 @@ SIGNATURES = {
     "npmg_snapshot_load": (_int, [_vp, _vp, _sz]),
     "npmg_pack_queries": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_f32), _u32, _vp, _vp]),
     "npmg_pack_training": (_int, [_vp, ctypes.POINTER(NpmgPathRecords), ctypes.POINTER(NpmgTrainTile),
-                                  ctypes.POINTER(_f32), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                  ctypes.POINTER(_f32), _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "npmg_sample_pdf_ex": (_int, [_vp, ctypes.POINTER(NpmgQueue), _u32, _u32, _u32, _vp]),
+    "npmg_train_step_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _int, _vp, _vp]),
+    "npmg_mixture_adjoint_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _vp, _vp, _vp]),
+    "npmg_selection_probability": (_int, [_vp, _vp, _vp, _u32, _vp]),
     "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 }
@@ -384,7 +390,7 @@ class GuidingNetwork:
                                                     _ptr(tgt_pdf), n, float(h), _ptr(out), _stream()))
         return out
 
-    def sample_pdf_queues(self, queues, counts, n_eval: int = 0, cosine_aware: bool = True):
+    def sample_pdf_queues(self, queues, counts, n_eval: int = 0, cosine_aware: bool = True, selection: bool = False):
         """sample_pdf over several queues in one persistent launch.
 
         queues: list of dicts with "inputs" [9][cap] and optionally "normal",
@@ -407,12 +413,62 @@ class GuidingNetwork:
                 o["lobe"] = torch.zeros(cap, dtype=torch.int32, device="cuda")
             if n_eval:
                 o["eval_pdf"] = torch.zeros(n_eval, cap, dtype=torch.float32, device="cuda")
+            if selection:
+                o["selection"] = torch.zeros(cap, dtype=torch.float32, device="cuda")
             arr[i] = NpmgQueue(_ptr(q["inputs"]), _ptr(q.get("normal")) if flags else None, _ptr(q.get("u")),
                                _ptr(q.get("eval_dirs")), _ptr(o.get("dir")), _ptr(o.get("pdf")),
-                               _ptr(o.get("eval_pdf")), _ptr(o.get("lobe")), cap)
+                               _ptr(o.get("eval_pdf")), _ptr(o.get("lobe")), cap, _ptr(o.get("selection")))
             outs.append(o)
         self._check(self._lib.npmg_sample_pdf_queues(self._h, arr, _ptr(counts), nq, n_eval, flags, _stream()))
         return outs
+
+    def sample_pdf_ex(self, inputs, normal=None, u=None, eval_dirs=None, cosine_aware: bool = True,
+                      selection: bool = True):
+        """sample_pdf through npmg_sample_pdf_ex, with the learned selection probability."""
+        import torch
+
+        n = inputs.shape[1]
+        n_eval = 0 if eval_dirs is None else eval_dirs.shape[0]
+        o = {}
+        if u is not None:
+            o["dir"] = torch.empty(3, n, device="cuda")
+            o["pdf"] = torch.empty(n, device="cuda")
+            o["lobe"] = torch.empty(n, dtype=torch.int32, device="cuda")
+        if n_eval:
+            o["eval_pdf"] = torch.empty(n_eval, n, device="cuda")
+        if selection:
+            o["selection"] = torch.empty(n, device="cuda")
+        flags = NPMG_COSINE_AWARE if (cosine_aware and normal is not None) else 0
+        q = NpmgQueue(_ptr(inputs), _ptr(normal) if flags else None, _ptr(u), _ptr(eval_dirs), _ptr(o.get("dir")),
+                      _ptr(o.get("pdf")), _ptr(o.get("eval_pdf")), _ptr(o.get("lobe")), n, _ptr(o.get("selection")))
+        self._check(self._lib.npmg_sample_pdf_ex(self._h, ctypes.byref(q), n, n_eval, flags, _stream()))
+        return o
+
+    def selection_probability(self, raw_out):
+        import torch
+
+        out = torch.empty(raw_out.shape[1], device="cuda")
+        self._check(self._lib.npmg_selection_probability(self._h, _ptr(raw_out), _ptr(out), raw_out.shape[1],
+                                                         _stream()))
+        return out
+
+    def train_step_selection(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf_pdf, loss_out=None, graph=False):
+        n = inputs.shape[1]
+        self._check(self._lib.npmg_train_step_selection(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb),
+                                                        _ptr(tgt_pdf), _ptr(tgt_bsdf_pdf), n, int(graph),
+                                                        _ptr(loss_out), _stream()))
+        return loss_out
+
+    def mixture_adjoint_selection(self, raw_out, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf_pdf, loss_scale=1.0):
+        import torch
+
+        n = raw_out.shape[1]
+        dout = torch.empty(self.padded_output, n, device="cuda")
+        loss = torch.empty(n, device="cuda")
+        self._check(self._lib.npmg_mixture_adjoint_selection(self._h, _ptr(raw_out), _ptr(tgt_dir), _ptr(tgt_rgb),
+                                                             _ptr(tgt_pdf), _ptr(tgt_bsdf_pdf), n, loss_scale,
+                                                             _ptr(dout), _ptr(loss), _stream()))
+        return dout, loss
 
     def autotune(self, n: int):
         """Times the query path's launch shapes for batches of about n; returns (chunks, blocks_per_cu)."""
@@ -432,24 +488,26 @@ class GuidingNetwork:
 
     def pack_training(self, records, tile, box, capacity):
         """generate_training_data: records = dict of device tensors named as the
-        npmg_path_records fields ([D][C][P] float, delta uint8, depth int32 [P]);
-        tile = dict(offset, stride, res, resolution) of pairs; box = 6 floats.
-        Returns (in [9][cap], tgt_dir [2][cap], tgt_rgb [3][cap], tgt_pdf [cap], count)."""
+        npmg_path_records fields ([D][C][P] float, delta uint8, depth int32 [P],
+        optional bsdf_pdf [D][P]); tile = dict(offset, stride, res, resolution)
+        of pairs; box = 6 floats. Returns (in [9][cap], tgt_dir [2][cap],
+        tgt_rgb [3][cap], tgt_pdf [cap], count) plus tgt_bsdf_pdf [cap] when
+        the records hold bsdf_pdf."""
         import torch
 
         d = "cuda"
         out = (torch.zeros(9, capacity, device=d), torch.zeros(2, capacity, device=d),
                torch.zeros(3, capacity, device=d), torch.zeros(capacity, device=d))
+        bsdf = torch.zeros(capacity, device=d) if records.get("bsdf_pdf") is not None else None
         count = torch.zeros(1, dtype=torch.int32, device=d)
-        r = NpmgPathRecords(*[_ptr(records[k]) for k in (
-            "L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo", "normal", "roughness",
-            "depth")], records["L"].shape[0], records["depth"].shape[0])
+        r = NpmgPathRecords(*[_ptr(records.get(k)) for k in NpmgPathRecords.FIELDS],
+                            records["L"].shape[0], records["depth"].shape[0])
         t = NpmgTrainTile((_u32 * 2)(*tile["offset"]), (_u32 * 2)(*tile["stride"]), (_u32 * 2)(*tile["res"]),
                           (_u32 * 2)(*tile["resolution"]))
         b = (_f32 * 6)(*box)
         self._check(self._lib.npmg_pack_training(self._h, ctypes.byref(r), ctypes.byref(t), b, capacity,
-                                                 *[_ptr(o) for o in out], _ptr(count), _stream()))
-        return out + (count,)
+                                                 *[_ptr(o) for o in out], _ptr(bsdf), _ptr(count), _stream()))
+        return out + (count,) + ((bsdf,) if bsdf is not None else ())
 
     def snapshot(self, with_optimizer: bool = True) -> bytes:
         """tcnn-format MessagePack snapshot (Trainer::serialize) as bytes."""

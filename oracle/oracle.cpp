@@ -615,9 +615,14 @@ void query_batch(const orc_cfg* c, const float* params, const float* in, const f
 // ---------------------------------------------------------------------------
 // Loss and adjoint: train.cu:69-124 (compute_dL_doutput_divergence),
 // distribution.h:143-181 (gradients_probability), train.h:212-223.
+// With c->learn_selection: compute_dL_doutput_with_selection (train.cu:126-172)
+// and MixedSGWithSelection::gradients_probability (distribution.h:216-231):
+// P = s (p + eps) + (1 - s) bsdf, loss -f/P log P, prefix -f/P^2, lobe
+// gradients times s, the selection slot LR_PROB (p - bsdf) s (1 - s)
+// (train.h:55, 61), and neither NaN zeroing nor the clamp.
 template <class T>
-void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3], float wipdf, T ls_over_n,
-                 T clamp_hi, T* g, T* loss) {
+void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3], float wipdf, float bsdf,
+                 T ls_over_n, T clamp_hi, T* g, T* loss) {
     const bool ref = c->numerics == ORC_NUM_REFERENCE;
     Mixture<T> m;
     build_mixture(c, x, m);
@@ -659,8 +664,19 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
         dp[s][3] = wv * L.kappa * (wi.y * cp - wi.x * sp) * st;
     }
     const T guide = prob + C<T>::eps;
-    const T prefix = -Li / wiPdf / guide * ls_over_n;
-    *loss = -Li / wiPdf * M<T>::log(guide);
+    const bool sel = c->learn_selection != 0;
+    T prefix, s_sel = 1;
+    if (sel) {
+        s_sel = (T)1 / ((T)1 + M<T>::exp(-x[4 * K]));
+        const T P = s_sel * guide + ((T)1 - s_sel) * (T)bsdf;
+        prefix = -Li / P / P * ls_over_n;
+        *loss = -Li / P * M<T>::log(P);
+        for (int s = 0; s < K; ++s)
+            for (int j = 0; j < 4; ++j) dp[s][j] = dp[s][j] * s_sel;
+    } else {
+        prefix = -Li / wiPdf / guide * ls_over_n;
+        *loss = -Li / wiPdf * M<T>::log(guide);
+    }
     const int n_out = orc_padded_output(c);
     for (int j = 0; j < n_out; ++j) g[j] = 0;
     for (int s = 0; s < K; ++s) {
@@ -691,6 +707,10 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
             g[4 * s + 3] = prefix * dp[s][0] * m.lobe[s].lambda;
         }
     }
+    if (sel) {
+        g[4 * K] = (T)0.01 * prefix * (prob - (T)bsdf) * (s_sel * ((T)1 - s_sel));
+        return;
+    }
     // train.cu:106-123: zero non-finite gradients, clamp from above.
     for (int j = 0; j < 4 * K; ++j) {
         if (!M<T>::finite(g[j])) g[j] = 0;
@@ -700,7 +720,7 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
 
 template <class T>
 void adjoint_batch(const orc_cfg* c, const T* raw, const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf,
-                   int n, T loss_scale, T* dout, T* loss) {
+                   const float* tgt_bsdf, int n, T loss_scale, T* dout, T* loss) {
     const int n_out = orc_padded_output(c);
     const T ls = loss_scale / (T)n;
     // The reference clamps the scaled gradient at loss scale 128 (train.cu:116-122,
@@ -711,7 +731,8 @@ void adjoint_batch(const orc_cfg* c, const T* raw, const float* tgt_dir, const f
     for (int q = 0; q < n; ++q) {
         for (int j = 0; j < n_out; ++j) x[j] = raw[(size_t)j * n + q];
         const float rgb[3] = {tgt_rgb[q], tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q]};
-        adjoint_one<T>(c, x.data(), (T)tgt_dir[q], (T)tgt_dir[(size_t)n + q], rgb, tgt_pdf[q], ls, clamp_hi, g.data(), &loss[q]);
+        adjoint_one<T>(c, x.data(), (T)tgt_dir[q], (T)tgt_dir[(size_t)n + q], rgb, tgt_pdf[q],
+                       tgt_bsdf ? tgt_bsdf[q] : 0.0f, ls, clamp_hi, g.data(), &loss[q]);
         for (int j = 0; j < n_out; ++j) dout[(size_t)j * n + q] = g[j];
     }
 }
@@ -955,8 +976,9 @@ void orc_pack_queries_f(const float* box, const float* pos, const float* wo, con
 uint32_t orc_pack_training_f(const float* L, const float* thp, const float* pos, const float* dir,
                              const float* wi_pdf, const float* mis, const uint8_t* delta, const float* bsdf,
                              const float* wo, const float* normal, const float* rough, const uint32_t* depth,
-                             uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile, const float* box,
-                             uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf) {
+                             const float* bsdf_pdf, uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile,
+                             const float* box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
+                             float* tgt_pdf, float* tgt_bsdf_pdf) {
     const PackRecords<float> r{L, thp, pos, dir, wi_pdf, mis, bsdf, wo, normal, rough, delta, n_pixels};
     // tile: offset xy, stride xy, training resolution xy, image resolution xy
     const uint64_t n_threads = (uint64_t)tile[4] * tile[5];
@@ -977,6 +999,7 @@ uint32_t orc_pack_training_f(const float* L, const float* thp, const float* pos,
                 tgt_dir[C + count] = dr[1];
                 for (int k = 0; k < 3; ++k) tgt_rgb[(size_t)k * C + count] = rgb[k];
                 tgt_pdf[count] = pdf;
+                if (tgt_bsdf_pdf) tgt_bsdf_pdf[count] = bsdf_pdf ? bsdf_pdf[(size_t)d * n_pixels + pid] : 0.0f;
             }
             ++count;
         }
@@ -1077,11 +1100,25 @@ void orc_query_d(const orc_cfg* c, const float* params, const float* in, const f
 
 void orc_adjoint_f(const orc_cfg* c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
                    const float* tgt_pdf, int n, float loss_scale, float* dout, float* loss) {
-    adjoint_batch<float>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, n, loss_scale, dout, loss);
+    adjoint_batch<float>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, nullptr, n, loss_scale, dout, loss);
 }
 void orc_adjoint_d(const orc_cfg* c, const double* raw, const float* tgt_dir, const float* tgt_rgb,
                    const float* tgt_pdf, int n, double loss_scale, double* dout, double* loss) {
-    adjoint_batch<double>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, n, loss_scale, dout, loss);
+    adjoint_batch<double>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, nullptr, n, loss_scale, dout, loss);
+}
+
+void orc_adjoint_sel_f(const orc_cfg* c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                       const float* tgt_pdf, const float* tgt_bsdf, int n, float loss_scale, float* dout, float* loss) {
+    adjoint_batch<float>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, n, loss_scale, dout, loss);
+}
+void orc_adjoint_sel_d(const orc_cfg* c, const double* raw, const float* tgt_dir, const float* tgt_rgb,
+                       const float* tgt_pdf, const float* tgt_bsdf, int n, double loss_scale, double* dout,
+                       double* loss) {
+    adjoint_batch<double>(c, raw, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, n, loss_scale, dout, loss);
+}
+void orc_selection_f(const orc_cfg* c, const float* raw, int n, float* out) {
+    const size_t j = 4 * (size_t)c->n_lobes;
+    for (int q = 0; q < n; ++q) out[q] = 1.0f / (1.0f + fm::exp(-raw[j * n + q]));
 }
 
 void orc_mlp_backward_f(const orc_cfg* c, const float* w, const float* enc, const float* acts, const float* dout,

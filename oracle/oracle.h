@@ -49,6 +49,7 @@ typedef struct {
     float grad_clamp;               /* 0 disables the upper clamp */
     float clamp_scale;              /* loss scale at which grad_clamp applies (the reference's 128):
                                        at loss scale s the threshold is grad_clamp * s / clamp_scale */
+    int learn_selection;            /* 1: output 4 K is the selection logit (MixedSGWithSelection) */
 } orc_cfg;
 
 /* Layout of the network: sizes of the padded encoding, MLP and grid. */
@@ -84,6 +85,15 @@ void orc_query_d(const orc_cfg* c, const float* params, const float* in, const f
                  const float* u, const float* eval_dirs, int n_eval, int n, double* out_dir,
                  double* out_pdf, double* out_eval_pdf, uint32_t* out_lobe, double* raw_out, int n_threads);
 
+/* Loss and adjoint with the learned selection probability (train.cu:126-172):
+ * tgt_bsdf [n] BSDF pdfs. */
+void orc_adjoint_sel_f(const orc_cfg* c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
+                       const float* tgt_pdf, const float* tgt_bsdf, int n, float loss_scale, float* dout, float* loss);
+void orc_adjoint_sel_d(const orc_cfg* c, const double* raw, const float* tgt_dir, const float* tgt_rgb,
+                       const float* tgt_pdf, const float* tgt_bsdf, int n, double loss_scale, double* dout,
+                       double* loss);
+/* Selection probability logistic(raw[4 K]) (distribution.h:233-237). */
+void orc_selection_f(const orc_cfg* c, const float* raw, int n, float* out);
 /* KL loss and its adjoint with respect to the raw outputs (loss_scale already
  * divided by nothing: the routine divides by n like the reference). */
 void orc_adjoint_f(const orc_cfg* c, const float* raw, const float* tgt_dir, const float* tgt_rgb,
@@ -125,8 +135,9 @@ void orc_pack_queries_f(const float* box, const float* pos, const float* wo, con
 uint32_t orc_pack_training_f(const float* L, const float* thp, const float* pos, const float* dir,
                              const float* wi_pdf, const float* mis, const uint8_t* delta, const float* bsdf,
                              const float* wo, const float* normal, const float* rough, const uint32_t* depth,
-                             uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile, const float* box,
-                             uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb, float* tgt_pdf);
+                             const float* bsdf_pdf, uint32_t max_depth, uint32_t n_pixels, const uint32_t* tile,
+                             const float* box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
+                             float* tgt_pdf, float* tgt_bsdf_pdf);
 /* The analytic radiance field f(x, w) used for the training targets. */
 void orc_field_eval(uint64_t seed, const float* pos, const float* dir, int n, double* f);
 

@@ -30,6 +30,7 @@ class OrcCfg(ctypes.Structure):
         ("width", ctypes.c_int), ("n_hidden_layers", ctypes.c_int), ("n_output_dims", ctypes.c_int),
         ("mixture", ctypes.c_int), ("n_lobes", ctypes.c_int), ("mapping", ctypes.c_int), ("numerics", ctypes.c_int),
         ("cosine_kappa", ctypes.c_float), ("grad_clamp", ctypes.c_float), ("clamp_scale", ctypes.c_float),
+        ("learn_selection", ctypes.c_int),
     ]
 
 
@@ -72,7 +73,8 @@ def make_cfg(config: dict) -> OrcCfg:
     c.n_hidden_layers = int(net.get("n_hidden_layers", 2))
     mix = config.get("mixture", {})
     c.n_lobes = int(mix.get("n_lobes", 8))
-    c.n_output_dims = 4 * c.n_lobes
+    c.learn_selection = 1 if mix.get("learn_selection", False) else 0
+    c.n_output_dims = 4 * c.n_lobes + c.learn_selection
     c.mixture = 0
     c.mapping = 0 if mix.get("mapping", "reference").lower() == "reference" else 1
     c.numerics = 0 if mix.get("numerics", "stable").lower() == "reference" else 1
@@ -123,8 +125,11 @@ class Oracle:
         L.orc_field_eval.argtypes = [ctypes.c_uint64, _vp, _vp, ctypes.c_int, _vp]
         L.orc_init_params.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int, _vp]
         L.orc_pack_queries_f.argtypes = [_vp] * 5 + [ctypes.c_int, _vp]
-        L.orc_pack_training_f.argtypes = [_vp] * 12 + [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
-                                                        ctypes.c_uint32] + [_vp] * 4
+        L.orc_pack_training_f.argtypes = [_vp] * 13 + [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
+                                                        ctypes.c_uint32] + [_vp] * 5
+        L.orc_adjoint_sel_f.argtypes = [_vp] * 6 + [ctypes.c_int, _f, _vp, _vp]
+        L.orc_adjoint_sel_d.argtypes = [_vp] * 6 + [ctypes.c_int, ctypes.c_double, _vp, _vp]
+        L.orc_selection_f.argtypes = [_vp, _vp, ctypes.c_int, _vp]
         L.orc_pack_training_f.restype = ctypes.c_uint32
 
     # -- layout -----------------------------------------------------------
@@ -230,6 +235,24 @@ class Oracle:
            k(c32(tgt_pdf)), n, loss_scale, _p(dout), _p(loss))
         return dout, loss
 
+    def adjoint_selection(self, cfg, raw, tgt_dir, tgt_rgb, tgt_pdf, tgt_bsdf, loss_scale=1.0, dtype=np.float32):
+        k = _Keep()
+        n = raw.shape[1]
+        _, po = self.widths(cfg)
+        dout = np.zeros((po, n), dtype)
+        loss = np.zeros(n, dtype)
+        fn = self.lib.orc_adjoint_sel_f if dtype == np.float32 else self.lib.orc_adjoint_sel_d
+        c32 = lambda a: np.ascontiguousarray(a, np.float32)
+        fn(ctypes.byref(cfg), k(raw, dtype), k(c32(tgt_dir)), k(c32(tgt_rgb)), k(c32(tgt_pdf)), k(c32(tgt_bsdf)),
+           n, loss_scale, _p(dout), _p(loss))
+        return dout, loss
+
+    def selection(self, cfg, raw):
+        k = _Keep()
+        out = np.zeros(raw.shape[1], np.float32)
+        self.lib.orc_selection_f(ctypes.byref(cfg), k(raw, np.float32), raw.shape[1], _p(out))
+        return out
+
     def mlp_backward(self, cfg, mlp_params, enc, acts, dout):
         k = _Keep()
         dtype = enc.dtype
@@ -289,21 +312,23 @@ class Oracle:
         return out
 
     PACK_FIELDS = ("L", "thp", "pos", "dir", "wi_pdf", "mis_weight", "delta", "bsdf_val", "wo", "normal",
-                   "roughness", "depth")
+                   "roughness", "depth", "bsdf_pdf")
 
     def pack_training(self, rec, tile, box, capacity):
-        """rec: dict of numpy arrays [D][C][P] (delta uint8, depth uint32 [P]);
-        tile: dict(offset, stride, res, resolution). Returns (in, dir, rgb, pdf, count)."""
+        """rec: dict of numpy arrays [D][C][P] (delta uint8, depth uint32 [P],
+        optional bsdf_pdf [D][P]); tile: dict(offset, stride, res, resolution).
+        Returns (in, dir, rgb, pdf, count) plus bsdf_pdf when rec has it."""
         k = _Keep()
-        ptrs = [k(rec[f], np.uint8 if f == "delta" else (np.uint32 if f == "depth" else np.float32))
+        ptrs = [k(rec.get(f), np.uint8 if f == "delta" else (np.uint32 if f == "depth" else np.float32))
                 for f in self.PACK_FIELDS]
         t = np.array(list(tile["offset"]) + list(tile["stride"]) + list(tile["res"]) + list(tile["resolution"]),
                      np.uint32)
         out = (np.zeros((9, capacity), np.float32), np.zeros((2, capacity), np.float32),
                np.zeros((3, capacity), np.float32), np.zeros(capacity, np.float32))
+        bsdf = np.zeros(capacity, np.float32) if rec.get("bsdf_pdf") is not None else None
         cnt = self.lib.orc_pack_training_f(*ptrs, rec["L"].shape[0], rec["depth"].shape[0], k(t, np.uint32),
-                                           k(box, np.float32), capacity, *[_p(o) for o in out])
-        return out + (int(cnt),)
+                                           k(box, np.float32), capacity, *[_p(o) for o in out], _p(bsdf))
+        return out + (int(cnt),) + ((bsdf,) if bsdf is not None else ())
 
     def field_eval(self, seed, pos, dirs):
         k = _Keep()
