@@ -258,6 +258,35 @@ npmg_status npmg_pack_training(npmg_ctx* ctx, const npmg_path_records* records, 
                                const float* scene_box, uint32_t capacity, float* in, float* tgt_dir, float* tgt_rgb,
                                float* tgt_pdf, float* tgt_bsdf_pdf, uint32_t* count, npmg_stream stream);
 
+/* ---- Training policy (host side of trainStep, integrator.cpp:595-702) ---- */
+
+/* updateTrainingStride (integrator.cpp:595-625): the pixel stride of the next
+ * frame's training tile from this frame's record count, so that about
+ * batch_size * batches_per_frame records are produced (the target tile area
+ * is capped at 2^22 pixels). cur, min_stride,
+ * out: host uint32[2]. Returns 1 if the reference would accept the change
+ * (out differs from cur as its unsigned test decides), else 0 (out = cur). */
+int npmg_training_stride(const uint32_t* cur, const uint32_t* min_stride, uint32_t batch_size,
+                         uint32_t batches_per_frame, uint32_t n_training_samples, uint32_t* out);
+/* The batch loop of trainStep (integrator.cpp:666-672): min(n / batch_size + 1,
+ * batches_per_frame) batches at offsets i * batch_size of min(n - offset,
+ * batch_size) records each, rounded down to a multiple of granularity (the
+ * reference's 128, tcnn's batch rule; 1 here needs none), stopping at the
+ * first batch below min_batch (MIN_TRAIN_BATCH_SIZE, parameters.h:19).
+ * offsets, sizes: host arrays of max_batches; *n_batches: batches written. */
+npmg_status npmg_train_batches(uint32_t n_samples, uint32_t batch_size, uint32_t batches_per_frame,
+                               uint32_t granularity, uint32_t min_batch, uint32_t* offsets, uint32_t* sizes,
+                               uint32_t max_batches, uint32_t* n_batches);
+/* trainStep's loop over packed records (npmg_pack_training output, row
+ * stride `stride`, n_records kept): each batch of npmg_train_batches is
+ * copied to a contiguous staging buffer and trained with npmg_train_step
+ * (npmg_train_step_selection when tgt_bsdf_pdf is given); batch b's mean loss
+ * goes to losses[b] (device, may be NULL). *n_batches (host): batches run. */
+npmg_status npmg_train_records(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                               const float* tgt_pdf, const float* tgt_bsdf_pdf, uint32_t stride, uint32_t n_records,
+                               uint32_t batch_size, uint32_t batches_per_frame, uint32_t granularity,
+                               uint32_t min_batch, float* losses, uint32_t* n_batches, npmg_stream stream);
+
 /* ---- Split passes (test harness and autograd wrapping) ------------------ */
 
 /* Encoding only: in [9][n] -> enc [encoded_width][n]. */

@@ -201,6 +201,8 @@ struct npmg_ctx {
     uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
     uint32_t* pack_scratch = nullptr;   // block totals of the training packer
     size_t pack_cap = 0;
+    float* stage = nullptr;  // contiguous batch of npmg_train_records: in [9], dir [2], rgb [3], pdf, bsdf pdf
+    uint32_t stage_cap = 0;
     // FP8 inference: calibrated input maxima per layer and the scales derived from them
     // (wscale [L], ascale [L], qscale [L + 1]); calibration uses at most kFp8CalibMax queries
     static constexpr uint32_t kFp8CalibMax = 1u << 16;
@@ -245,7 +247,8 @@ struct npmg_ctx {
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
                         (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes,
-                        (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales, (void*)pack_scratch})
+                        (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales, (void*)pack_scratch,
+                        (void*)stage})
             if (p) (void)hipFree(p);
     }
 
@@ -1440,6 +1443,91 @@ npmg_status npmg_train_step(npmg_ctx* ctx, const float* in, const float* tgt_dir
     if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
         train_step_entry(ctx, in, tgt_dir, tgt_rgb, tgt_pdf, nullptr, n, loss_out, false, S(stream));
+    });
+}
+
+int npmg_training_stride(const uint32_t* cur, const uint32_t* min_stride, uint32_t batch_size,
+                         uint32_t batches_per_frame, uint32_t n_training_samples, uint32_t* out) {
+    if (!cur || !min_stride || !out) return 0;
+    // the reference's float arithmetic and unsigned comparisons, in its order
+    const float tile = float(cur[0] * cur[1]);
+    const float utilization = float(batch_size * batches_per_frame) / float(n_training_samples);
+    // (bounded: with no batch budget the reference's loop below would not end)
+    const float period = std::min(tile / utilization, 4194304.0f);
+    const uint32_t side = std::max(1u, uint32_t(std::sqrt(std::max(0.f, period))));
+    uint32_t x = std::max(side, min_stride[0]), y = std::max(side, min_stride[1]);
+    while (period - float((x + 1) * y) >= 0) x += 1;
+    bool accept = x * y < cur[0] * cur[1];
+    accept |= std::max(x - cur[0], 0u) + std::max(y - cur[1], 0u) >= 1;
+    out[0] = accept ? x : cur[0];
+    out[1] = accept ? y : cur[1];
+    return accept ? 1 : 0;
+}
+
+npmg_status npmg_train_batches(uint32_t n_samples, uint32_t batch_size, uint32_t batches_per_frame,
+                               uint32_t granularity, uint32_t min_batch, uint32_t* offsets, uint32_t* sizes,
+                               uint32_t max_batches, uint32_t* n_batches) {
+    if (!n_batches || batch_size == 0 || granularity == 0 || (max_batches && !(offsets && sizes))) return NPMG_EINVAL;
+    const uint32_t count = std::min(n_samples / batch_size + 1u, batches_per_frame);
+    uint32_t k = 0;
+    for (uint32_t it = 0; it < count; ++it) {
+        uint32_t local = std::min(n_samples - it * batch_size, batch_size);
+        local -= local % granularity;
+        if (local < min_batch || local == 0) break;
+        if (k >= max_batches) return NPMG_EINVAL;
+        offsets[k] = it * batch_size;
+        sizes[k] = local;
+        ++k;
+    }
+    *n_batches = k;
+    return NPMG_OK;
+}
+
+npmg_status npmg_train_records(npmg_ctx* ctx, const float* in, const float* tgt_dir, const float* tgt_rgb,
+                               const float* tgt_pdf, const float* tgt_bsdf_pdf, uint32_t stride, uint32_t n_records,
+                               uint32_t batch_size, uint32_t batches_per_frame, uint32_t granularity,
+                               uint32_t min_batch, float* losses, uint32_t* n_batches, npmg_stream stream) {
+    if (!ctx || !n_batches) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(n_records <= stride, "n_records exceeds the row stride");
+        need(in && tgt_dir && tgt_rgb && tgt_pdf, "null records");
+        need(!ctx->cfg.d.sel || tgt_bsdf_pdf, "learned selection needs tgt_bsdf_pdf");
+        std::vector<uint32_t> off(batches_per_frame ? batches_per_frame : 1), len(off.size());
+        uint32_t nb = 0;
+        need(npmg_train_batches(n_records, batch_size, batches_per_frame, granularity, min_batch, off.data(),
+                                len.data(), (uint32_t)off.size(), &nb) == NPMG_OK,
+             "invalid batch policy");
+        *n_batches = nb;
+        if (nb == 0) return;
+        const uint32_t cap = *std::max_element(len.begin(), len.begin() + nb);
+        if (cap > ctx->stage_cap) {
+            if (ctx->stage) hip_check(hipFree(ctx->stage), "hipFree");
+            ctx->stage = nullptr;
+            ctx->stage_cap = 0;
+            ctx->stage = dalloc<float>((size_t)16 * cap);
+            ctx->stage_cap = cap;
+        }
+        const hipStream_t s = S(stream);
+        for (uint32_t b = 0; b < nb; ++b) {
+            const uint32_t m = len[b], o = off[b];
+            float* st_in = ctx->stage;
+            float* st_dir = st_in + (size_t)9 * m;
+            float* st_rgb = st_dir + (size_t)2 * m;
+            float* st_pdf = st_rgb + (size_t)3 * m;
+            float* st_bsdf = st_pdf + m;
+            auto rows = [&](float* dst, const float* src, int r) {
+                hip_check(hipMemcpy2DAsync(dst, (size_t)m * 4, src + o, (size_t)stride * 4, (size_t)m * 4, (size_t)r,
+                                           hipMemcpyDeviceToDevice, s),
+                          "stage batch");
+            };
+            rows(st_in, in, 9);
+            rows(st_dir, tgt_dir, 2);
+            rows(st_rgb, tgt_rgb, 3);
+            rows(st_pdf, tgt_pdf, 1);
+            if (tgt_bsdf_pdf) rows(st_bsdf, tgt_bsdf_pdf, 1);
+            train_step_entry(ctx, st_in, st_dir, st_rgb, st_pdf, ctx->cfg.d.sel ? st_bsdf : nullptr, m,
+                             losses ? losses + b : nullptr, false, s);
+        }
     });
 }
 

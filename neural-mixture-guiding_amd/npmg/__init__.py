@@ -92,6 +92,12 @@ SIGNATURES = {
     "npmg_train_step_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _int, _vp, _vp]),
     "npmg_mixture_adjoint_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _vp, _vp, _vp]),
     "npmg_selection_probability": (_int, [_vp, _vp, _vp, _u32, _vp]),
+    "npmg_training_stride": (_int, [ctypes.POINTER(_u32), ctypes.POINTER(_u32), _u32, _u32, _u32,
+                                    ctypes.POINTER(_u32)]),
+    "npmg_train_batches": (_int, [_u32, _u32, _u32, _u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32), _u32,
+                                  ctypes.POINTER(_u32)]),
+    "npmg_train_records": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp,
+                                  ctypes.POINTER(_u32), _vp]),
     "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
 }
@@ -470,6 +476,16 @@ class GuidingNetwork:
                                                              _ptr(dout), _ptr(loss), _stream()))
         return dout, loss
 
+    def train_records(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, n_records, batch_size=524288, batches_per_frame=8,
+                      granularity=128, min_batch=65536, tgt_bsdf_pdf=None, losses=None):
+        """trainStep's batch loop over packed records (row stride inputs.shape[1]); returns the batch count."""
+        nb = _u32()
+        self._check(self._lib.npmg_train_records(self._h, _ptr(inputs), _ptr(tgt_dir), _ptr(tgt_rgb), _ptr(tgt_pdf),
+                                                 _ptr(tgt_bsdf_pdf), inputs.shape[1], n_records, batch_size,
+                                                 batches_per_frame, granularity, min_batch, _ptr(losses),
+                                                 ctypes.byref(nb), _stream()))
+        return nb.value
+
     def autotune(self, n: int):
         """Times the query path's launch shapes for batches of about n; returns (chunks, blocks_per_cu)."""
         self._check(self._lib.npmg_autotune(self._h, n, _stream()))
@@ -588,6 +604,25 @@ def pack_queries(pos, wo, normal, roughness, box):
     if st != NPMG_OK:
         raise NpmgError(st, "npmg_pack_queries failed")
     return out
+
+
+def training_stride(cur, min_stride, batch_size, batches_per_frame, n_training_samples):
+    """updateTrainingStride: (accepted, new stride)."""
+    out = (_u32 * 2)()
+    acc = lib().npmg_training_stride((_u32 * 2)(*cur), (_u32 * 2)(*min_stride), batch_size, batches_per_frame,
+                                     n_training_samples, out)
+    return bool(acc), (out[0], out[1])
+
+
+def train_batches(n_samples, batch_size=524288, batches_per_frame=8, granularity=128, min_batch=65536):
+    """The trainStep batch policy: [(offset, size), ...]."""
+    m = max(batches_per_frame, 1)
+    off, size, nb = (_u32 * m)(), (_u32 * m)(), _u32()
+    st = lib().npmg_train_batches(n_samples, batch_size, batches_per_frame, granularity, min_batch, off, size, m,
+                                  ctypes.byref(nb))
+    if st != NPMG_OK:
+        raise NpmgError(st, "npmg_train_batches failed")
+    return [(off[i], size[i]) for i in range(nb.value)]
 
 
 def debug_math(fn: int, x, y=None):
