@@ -234,7 +234,7 @@ def main():
 
     def asg_cfg(precision):
         c = npmg.benchmark_config("full", precision=precision, width=128, hidden=3)
-        c["mixture"] = {"otype": "ASG", "n_lobes": 16}
+        c["mixture"] = {"otype": "ASG", "n_lobes": 16, "numerics": "fast"}
         return c
 
     def measure_frame(precision):

@@ -135,19 +135,42 @@ template <int N> __device__ __forceinline__ void s_sincos(const Dual<N>& x, Dual
     *c = dchain(cv, -sv, x);
 }
 
+// The scalar functions of the lobe code under a math policy: float values
+// through MT (ExactMath or FastMath, device.h), dual numbers always exact.
+template <class MT>
+struct Sx {
+    static __device__ __forceinline__ float exp(float x) { return MT::exp(x); }
+    static __device__ __forceinline__ float log(float x) { return MT::log(x); }
+    static __device__ __forceinline__ float expm1(float x) { return MT::expm1(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return MT::sqrt(x); }
+    static __device__ __forceinline__ float logistic(float x) { return MT::div(1.0f, 1.0f + MT::exp(-x)); }
+    static __device__ __forceinline__ void sincos(float x, float* s, float* c) { MT::sincos(x, s, c); }
+    static __device__ __forceinline__ float div(float a, float b) { return MT::div(a, b); }
+    template <int N> static __device__ __forceinline__ Dual<N> exp(const Dual<N>& x) { return s_exp(x); }
+    template <int N> static __device__ __forceinline__ Dual<N> log(const Dual<N>& x) { return s_log(x); }
+    template <int N> static __device__ __forceinline__ Dual<N> expm1(const Dual<N>& x) { return s_expm1(x); }
+    template <int N> static __device__ __forceinline__ Dual<N> sqrt(const Dual<N>& x) { return s_sqrt(x); }
+    template <int N> static __device__ __forceinline__ Dual<N> logistic(const Dual<N>& x) { return s_logistic(x); }
+    template <int N> static __device__ __forceinline__ void sincos(const Dual<N>& x, Dual<N>* s, Dual<N>* c) {
+        s_sincos(x, s, c);
+    }
+    template <int N> static __device__ __forceinline__ Dual<N> div(const Dual<N>& a, const Dual<N>& b) { return a / b; }
+};
+
 template <class S>
 struct AsgLobe {
     S z[3], t1[3], t2[3], kappa, a, lognorm;
 };
 
 // Lobe from its five shape outputs r = (x0 .. x4); S = float or Dual<5>.
-template <class S>
+template <class S, class MT = ExactMath>
 __device__ __forceinline__ AsgLobe<S> asg_build(const S (&r)[5]) {
+    using F = Sx<MT>;
     AsgLobe<S> L;
     S sth, cth, sph, cph, sps, cps;
-    s_sincos(kPi * s_logistic(r[0]), &sth, &cth);
-    s_sincos(kTwoPi * s_logistic(r[1]), &sph, &cph);
-    s_sincos(kPi * s_logistic(r[2]), &sps, &cps);
+    F::sincos(kPi * F::logistic(r[0]), &sth, &cth);
+    F::sincos(kTwoPi * F::logistic(r[1]), &sph, &cph);
+    F::sincos(kPi * F::logistic(r[2]), &sps, &cps);
     L.z[0] = sth * cph;
     L.z[1] = sth * sph;
     L.z[2] = cth;
@@ -159,7 +182,7 @@ __device__ __forceinline__ AsgLobe<S> asg_build(const S (&r)[5]) {
         L.t2[i] = cps * e2[i] - sps * e1[i];
     }
     // clamps: the clamped value with a zero derivative outside the range
-    const S ek = s_exp(r[3]);
+    const S ek = F::exp(r[3]);
     const float kv = s_val(ek);
     L.kappa = ek;
     if (!(kv >= 1e-2f)) L.kappa = s_lift(ek, 1e-2f);
@@ -167,23 +190,24 @@ __device__ __forceinline__ AsgLobe<S> asg_build(const S (&r)[5]) {
     S la = r[4];
     if (s_val(la) > kAsgLogAMax) la = s_lift(la, kAsgLogAMax);
     if (s_val(la) < -kAsgLogAMax) la = s_lift(la, -kAsgLogAMax);
-    L.a = s_exp(la);
+    L.a = F::exp(la);
     // log C(kappa) = log kappa + log(1/2pi) - log(-expm1(-2 kappa)), kappa >= 1e-2
-    L.lognorm = s_log(L.kappa) + s_lift(la, fm::log(kInv2Pi)) - s_log(-s_expm1((-2.0f) * L.kappa));
+    L.lognorm = F::log(L.kappa) + s_lift(la, fm::log(kInv2Pi)) - F::log(-F::expm1((-2.0f) * L.kappa));
     return L;
 }
 
 // D(w) of one lobe at a unit direction w.
-template <class S>
+template <class S, class MT = ExactMath>
 __device__ __forceinline__ S asg_eval(const AsgLobe<S>& L, V3 w) {
+    using F = Sx<MT>;
     const S xl = w.x * L.t1[0] + w.y * L.t1[1] + w.z * L.t1[2];
     const S yl = w.x * L.t2[0] + w.y * L.t2[1] + w.z * L.t2[2];
     const S zl = w.x * L.z[0] + w.y * L.z[1] + w.z * L.z[2];
-    const S ux = xl / L.a, uy = yl * L.a;
+    const S ux = F::div(xl, L.a), uy = yl * L.a;
     const S s2 = ux * ux + uy * uy + zl * zl;
-    S cm1 = zl / s_sqrt(s2) - s_lift(zl, 1.0f);  // cos - 1 in the unstretched frame, at most 0
+    S cm1 = F::div(zl, F::sqrt(s2)) - s_lift(zl, 1.0f);  // cos - 1 in the unstretched frame, at most 0
     if (s_val(cm1) > 0.0f) cm1 = s_lift(cm1, 0.0f);
-    return s_exp(L.kappa * cm1 + L.lognorm - 1.5f * s_log(s2));
+    return F::exp(L.kappa * cm1 + L.lognorm - 1.5f * F::log(s2));
 }
 
 // The shape outputs of lobe i as S, from a raw-output accessor.
@@ -194,7 +218,7 @@ __device__ __forceinline__ void asg_raw(Raw&& x, int i, float (&r)[5]) {
 }
 
 // Mixture weights: softmax over the logits (slot 5 of each lobe).
-template <int K, class Raw>
+template <int K, class MT = ExactMath, class Raw>
 __device__ __forceinline__ void asg_weights(Raw&& x, float (&w)[K]) {
     float mx = x(5);
 #pragma unroll
@@ -202,16 +226,16 @@ __device__ __forceinline__ void asg_weights(Raw&& x, float (&w)[K]) {
     float total = 0.0f;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-        w[i] = fm::exp(x(6 * i + 5) - mx);
+        w[i] = MT::exp(x(6 * i + 5) - mx);
         total += w[i];
     }
 #pragma unroll
-    for (int i = 0; i < K; ++i) w[i] = w[i] / total;
+    for (int i = 0; i < K; ++i) w[i] = MT::div(w[i], total);
 }
 
 // Lobe choice by running subtraction over the weights (the vMF mixture's rule,
 // distribution.h:132-140), then the lobe's own direction.
-template <int K, class Raw>
+template <int K, class MT = ExactMath, class Raw>
 __device__ __forceinline__ V3 asg_sample(Raw&& x, const float (&w)[K], float u0, float u1, float u2,
                                          uint32_t* lobe) {
     int sel = -1;
@@ -236,13 +260,13 @@ __device__ __forceinline__ V3 asg_sample(Raw&& x, const float (&w)[K], float u0,
         }
         r[j] = fm::bits_to_float(b);
     }
-    const AsgLobe<float> L = asg_build<float>(r);
-    const float ct = 1.0f + fm::log1p(u1 * fm::expm1(-2.0f * L.kappa)) / L.kappa;
-    const float st = fm::sqrt_(smax(0.0f, 1.0f - ct * ct));
+    const AsgLobe<float> L = asg_build<float, MT>(r);
+    const float ct = 1.0f + MT::div(fm::log1p(u1 * fm::expm1(-2.0f * L.kappa)), L.kappa);
+    const float st = MT::sqrt(smax(0.0f, 1.0f - ct * ct));
     float sp, cp;
-    fm::sincos(kTwoPi * u2, &sp, &cp);
-    const float vx = L.a * (cp * st), vy = (sp * st) / L.a, vz = ct;
-    const float inv = 1.0f / fm::sqrt_(vx * vx + vy * vy + vz * vz);
+    MT::sincos(kTwoPi * u2, &sp, &cp);
+    const float vx = L.a * (cp * st), vy = MT::div(sp * st, L.a), vz = ct;
+    const float inv = MT::div(1.0f, MT::sqrt(vx * vx + vy * vy + vz * vz));
     const float lx = vx * inv, ly = vy * inv, lz = vz * inv;
     return mk(lx * L.t1[0] + ly * L.t2[0] + lz * L.z[0], lx * L.t1[1] + ly * L.t2[1] + lz * L.z[1],
               lx * L.t1[2] + ly * L.t2[2] + lz * L.z[2]);
@@ -250,7 +274,7 @@ __device__ __forceinline__ V3 asg_sample(Raw&& x, const float (&w)[K], float u0,
 
 // Mixture pdf at up to E directions at once (bit e of `valid` set), one lobe
 // built at a time, so only one lobe's state is live.
-template <int K, int E, class Raw>
+template <int K, int E, class MT = ExactMath, class Raw>
 __device__ __forceinline__ void asg_pdf(Raw&& x, const float (&w)[K], const V3 (&dirs)[E], uint32_t valid,
                                         float (&p)[E]) {
 #pragma unroll
@@ -259,25 +283,25 @@ __device__ __forceinline__ void asg_pdf(Raw&& x, const float (&w)[K], const V3 (
     for (int i = 0; i < K; ++i) {
         float r[5];
         asg_raw(x, i, r);
-        const AsgLobe<float> L = asg_build<float>(r);
+        const AsgLobe<float> L = asg_build<float, MT>(r);
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            if ((valid >> e) & 1u) p[e] = fm::fma_(w[i], asg_eval<float>(L, dirs[e]), p[e]);
+            if ((valid >> e) & 1u) p[e] = fm::fma_(w[i], asg_eval<float, MT>(L, dirs[e]), p[e]);
     }
 }
 
 // One query's sample and pdfs from its raw outputs: direction, its pdf and
 // the lobe when io.u is set, and the pdf at each evaluation direction (four
 // per pass over the lobes; the sample direction joins the first pass).
-template <int K, class Raw>
-__device__ __forceinline__ void asg_query(Raw&& x, const QueryIO& io, uint32_t q, uint32_t n) {
+template <int K, class MT, class Raw>
+__device__ __forceinline__ void asg_query_m(Raw&& x, const QueryIO& io, uint32_t q, uint32_t n) {
     float w[K];
-    asg_weights<K>(x, w);
+    asg_weights<K, MT>(x, w);
     V3 dirs[5];
     uint32_t valid = 0u;
     if (io.u) {
         uint32_t lobe;
-        dirs[0] = asg_sample<K>(x, w, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
+        dirs[0] = asg_sample<K, MT>(x, w, io.u[q], io.u[(size_t)n + q], io.u[2 * (size_t)n + q], &lobe);
         io.out_dir[q] = dirs[0].x;
         io.out_dir[(size_t)n + q] = dirs[0].y;
         io.out_dir[2 * (size_t)n + q] = dirs[0].z;
@@ -299,13 +323,20 @@ __device__ __forceinline__ void asg_query(Raw&& x, const QueryIO& io, uint32_t q
             }
         }
         float p[5];
-        asg_pdf<K, 5>(x, w, dirs, valid, p);
+        asg_pdf<K, 5, MT>(x, w, dirs, valid, p);
         if (valid & 1u) io.out_pdf[q] = p[0];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if ((valid >> (1 + k)) & 1u) io.out_eval_pdf[(size_t)(base + k) * n + q] = p[1 + k];
         valid = 0u;
     }
+}
+
+// asg_query_m with the math policy of the configuration's numerics.
+template <int K, class Raw>
+__device__ __forceinline__ void asg_query(const DevCfg& c, Raw&& x, const QueryIO& io, uint32_t q, uint32_t n) {
+    if (c.numerics == NUM_FAST) asg_query_m<K, FastMath>(x, io, q, n);
+    else asg_query_m<K, ExactMath>(x, io, q, n);
 }
 
 // Loss -(f/q) log(p + eps) and its adjoint for the anisotropic mixture (the

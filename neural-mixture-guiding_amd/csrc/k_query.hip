@@ -82,7 +82,7 @@ __device__ __forceinline__ void mlp_forward(const DevCfg& c, const float* __rest
 template <int K, int P, int NOUT>
 __device__ __forceinline__ void mixture_query(const DevCfg& c, const float* out, const QueryIO& io, uint32_t q,
                                               uint32_t n) {
-    if constexpr (P == 6) asg_query<K>([&](int j) { return out[j]; }, io, q, n);
+    if constexpr (P == 6) asg_query<K>(c, [&](int j) { return out[j]; }, io, q, n);
     else vmf_query_cfg<K, true>(c, [&](int j) { return out[j]; }, io, q, n);
 }
 

@@ -158,7 +158,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
             for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
         }
         if constexpr (P == 6) {
-            if (io.u || io.n_eval > 0) asg_query<K>([&](int jj) { return out[jj]; }, io, q, n);
+            if (io.u || io.n_eval > 0) asg_query<K>(c, [&](int jj) { return out[jj]; }, io, q, n);
         } else if (io.u || io.n_eval > 0 || io.out_sel) {
             vmf_query_cfg<K, false>(c, [&](int jj) { return out[jj]; }, io, q, n);
         }

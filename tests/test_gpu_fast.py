@@ -110,3 +110,39 @@ def test_fast_samples_follow_pdf(gpu):
     print(f"fast bf16: chi2 {chi2:.0f} dof {int(keep.sum())} p {p:.3f}")
     assert p > 1e-4
     net.close()
+
+
+def test_fast_anisotropic_lobes(gpu):
+    """The anisotropic mixture's fast numerics against its exact functions on
+    the same raw outputs (the exact path is pinned by test_gpu_asg.py), and
+    the fused BF16 path against the consumer kernel bit for bit."""
+    import torch
+    import npmg
+
+    def cfg(num, prec="fp32"):
+        c = npmg.benchmark_config("small", precision=prec, width=128, hidden=3)
+        c["mixture"] = {"otype": "ASG", "n_lobes": 16, "numerics": num}
+        return c
+
+    exact = npmg.GuidingNetwork(cfg("stable"), seed=3)
+    fast = npmg.GuidingNetwork(cfg("fast"), seed=3)
+    n = 8192
+    q = npmg.synth_queries(5, n)
+    raw = torch.from_numpy(np.random.default_rng(1).normal(0, 1.5, (96, n)).astype(np.float32)).cuda()
+    a = exact.mixture_sample_pdf(raw, None, q["u"], q["eval_dirs"], cosine_aware=False)
+    b = fast.mixture_sample_pdf(raw, None, q["u"], q["eval_dirs"], cosine_aware=False)
+    ea, eb = a["eval_pdf"].cpu().numpy(), b["eval_pdf"].cpu().numpy()
+    ok = ea > 1e-30
+    rel = np.abs(eb - ea)[ok] / ea[ok]
+    print(f"anisotropic fast vs exact: eval pdf rel max {rel.max():.2e} median {np.median(rel):.2e}")
+    assert rel.max() < 2e-4  # measured 5e-5: |exponent| ~ kappa up to 1e3 scales the exp2 argument's rounding
+    same = (a["lobe"] == b["lobe"]).float().mean().item()
+    assert same > 0.999
+    exact.close()
+    fast.close()
+    net = npmg.GuidingNetwork(cfg("fast", "bf16"), seed=3)
+    out = net.sample_pdf(q["inputs"], None, q["u"], q["eval_dirs"], cosine_aware=False, raw=True)
+    c = net.mixture_sample_pdf(out["raw"], None, q["u"], q["eval_dirs"], cosine_aware=False)
+    for k in ("dir", "pdf", "eval_pdf", "lobe"):
+        assert np.array_equal(out[k].cpu().numpy().view(np.uint32), c[k].cpu().numpy().view(np.uint32)), k
+    net.close()
