@@ -28,6 +28,7 @@ N_TRAIN = 1 << 18
 FRAME_PATHS = 1920 * 1080  # one wavefront bounce of a 1920x1080 frame
 FRAME_BOUNCES = 8
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave-instructions/s: 256 CUs x 4 SIMDs, 64 lanes over 16-wide SIMDs
 FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 # in, u, normal, 4 eval dirs; dir, pdf, 4 pdfs, lobe
@@ -43,6 +44,18 @@ def profiled_traffic(precision):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def profiled_valu(precision):
+    """VALU wave-instructions per launch of the fused query kernel from the
+    committed rocprofv3 SQ counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_query_{precision}_full_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("valu_wave_instructions_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -77,18 +90,21 @@ def cpu_baseline(cfg, params, q_host, budget_s=10.0):
     oc = pyoracle.make_cfg(cfg)
     threads = max(1, min(16, os.cpu_count() or 1))
     n = 32768
-    done, elapsed = 0, 0.0
-    while elapsed < budget_s and done + n <= N_QUERIES:
-        sl = slice(done, done + n)
+    done, elapsed, start = 0, 0.0, 0
+    while elapsed < budget_s:
+        if start + n > N_QUERIES:  # wrap around the benchmark queries
+            start = 0
+        sl = slice(start, start + n)
         args = [np.ascontiguousarray(q_host[k][..., sl]) for k in ("inputs", "normal", "u", "eval_dirs")]
         t0 = time.perf_counter()
         o.query(oc, params, *args, n_threads=threads)
         elapsed += time.perf_counter() - t0
         done += n
+        start += n
         n = min(n * 2, 1 << 19)
     return {"value": done / elapsed, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of the 2^21 benchmark queries (FP32 oracle, encode+MLP+mixture+sample+4 pdfs), "
-                      f"{elapsed:.1f} s on {threads} threads"}
+            "sample": f"{done} of the 2^21 benchmark queries in order, wrapping (FP32 oracle with stable numerics, "
+                      f"encode+MLP+mixture+sample+4 pdfs), {elapsed:.1f} s on {threads} threads"}
 
 
 def main():
@@ -344,6 +360,15 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "avg_launch_ms": kq["avg_ms_per_launch"], "launches_per_step": kq["launches_per_step"]},
         "kernel_launches": per_kernel,
+        # the kernel's actual limiter: VALU issue (4 cycles per 64-lane FP32 instruction per SIMD,
+        # 1024 SIMDs at 2.4 GHz = 614.4 G wave-instructions/s; transcendentals issue at quarter rate)
+        "valu_issue": (lambda v: None if v is None else {
+            "wave_instructions_per_launch": v,
+            "achieved_per_s": v / (kq["avg_ms_per_launch"] * 1e-3),
+            "peak_per_s": VALU_ISSUE_PEAK,
+            "frac": v / (kq["avg_ms_per_launch"] * 1e-3) / VALU_ISSUE_PEAK,
+            "source": "profiles/k_query_%s_full_counters.md (SQ_INSTS_VALU)" % args.precision})(
+            profiled_valu(args.precision)),
         "mlp_compute": {"flops_per_launch": flops,
                         "achieved_tflops": flops / (kq["avg_ms_per_launch"] * 1e-3) / 1e12,
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},
