@@ -700,6 +700,30 @@ namespace dev {
 // Sum of the three wrapped quartic CDFs at bin boundary b (oneblob.h:114-123).
 __device__ __forceinline__ float oneblob_cdf(float b, float x, float fnb) { return oneblob_edge(b - x, fnb); }
 
+// Bins k0 .. k0 + 7 of one OneBlob dimension for the fast numerics. With
+// d = edge - x in [-1, 1], exactly one of d - 1, d, d + 1 lies within half a
+// period of 0, namely d - rint(d); the other two copies lie at least 8 kernel
+// radii away and contribute exactly 0 or 1, rint(d) + 1 in total. So the edge
+// value is P(nb (d - rint(d))) + rint(d) + 1 and a bin, the difference of two
+// edges, is dP + d(rint): the same quantity as the three-copy sum, without its
+// cancellation against the integer parts. P is the quartic CDF with the
+// polynomial in FMA form, clamped by evaluating at |u| <= 1.0625.
+__device__ __forceinline__ void oneblob_bins_fast(float xv, int k0, float fnb, float inv_nb, float* v) {
+    float P[9], R[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float d = (float)(k0 + i) * inv_nb - xv;
+        const float r = __builtin_rintf(d);
+        const float u = __builtin_amdgcn_fmed3f((d - r) * fnb, -1.0625f, 1.0625f);
+        const float u2 = u * u;
+        const float poly = __builtin_fmaf(u2, __builtin_fmaf(u2, 0.2f * 0.9375f, -2.0f / 3.0f * 0.9375f), 0.9375f);
+        P[i] = __builtin_amdgcn_fmed3f(__builtin_fmaf(u, poly, 0.5f), 0.0f, 1.0f);
+        R[i] = r;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (P[i + 1] - P[i]) + (R[i + 1] - R[i]);
+}
+
 // Features of one grid level (F values) accumulated over the corners in order.
 template <class T, int F>
 __device__ __forceinline__ void grid_level_features(const DevCfg& c, const T* __restrict__ table, int l, float x0,
@@ -775,7 +799,7 @@ __device__ __forceinline__ float single_feature(const DevCfg& c, const T* __rest
 template <class T>
 __device__ __forceinline__ void encode_chunk(const DevCfg& c, const T* __restrict__ table, const float* q_in, int ch,
                                              float v[8], const T* __restrict__ genc = nullptr, uint32_t q = 0,
-                                             uint32_t n = 0) {
+                                             uint32_t n = 0, bool fast = false) {
     const int f0 = 8 * ch;
     if (genc) {
         const int g0 = c.enc_offset[c.grid_enc], gw = c.n_levels * c.F;
@@ -818,6 +842,10 @@ __device__ __forceinline__ void encode_chunk(const DevCfg& c, const T* __restric
         const int d = r0 / nb, k0 = r0 % nb;
         const float xv = pick(q_in, b + d);
         const float inv_nb = 1.0f / fnb;  // power of two: exact
+        if (fast) {
+            oneblob_bins_fast(xv, k0, fnb, inv_nb, v);
+            return;
+        }
         float left = oneblob_cdf((float)k0 * inv_nb, xv, fnb);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {

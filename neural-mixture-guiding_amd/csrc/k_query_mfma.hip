@@ -16,11 +16,11 @@ namespace npmg {
 
 // Fused-kernel launchers for one shape, defined in query_mfma.h and
 // instantiated in k_query_mfma_inst.hip.
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const QueryIO& io,
                          uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, int bpc, const float* qscale,
                          hipStream_t s);
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 hipError_t launch_mfma_queues_t(const DevCfg& c, const void* wfrag, const void* table, const QueueSet& qs, int n_cu,
                                 int blocks_per_cu, const float* qscale, hipStream_t s);
 
@@ -223,7 +223,9 @@ hipError_t launch_mfma_queues_type(const DevCfg& c, const void* wfrag, const voi
                                    int n_cu, int bpc, const float* qscale, hipStream_t s) {
 #define NPMG_QQ(WW, KK, PP)                                \
     if (c.width == WW && c.K == KK && c.lobe_params == PP) \
-        return launch_mfma_queues_t<T, O, WW, KK, PP>(c, wfrag, table, qs, n_cu, bpc, qscale, s);
+        return c.numerics == NUM_FAST                                                                 \
+                   ? launch_mfma_queues_t<T, O, WW, KK, PP, true>(c, wfrag, table, qs, n_cu, bpc, qscale, s)     \
+                   : launch_mfma_queues_t<T, O, WW, KK, PP, false>(c, wfrag, table, qs, n_cu, bpc, qscale, s);
     NPMG_QQ(64, 7, 4)
     NPMG_QQ(64, 8, 4)
     NPMG_QQ(64, 16, 4)
@@ -242,7 +244,11 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
                             const float* qscale, hipStream_t s) {
 #define NPMG_QM(WW, KK, PP)                                      \
     if (c.width == WW && c.K == KK && c.lobe_params == PP)       \
-        return launch_mfma_t<T, O, WW, KK, PP>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, bpc, qscale, s);
+        return c.numerics == NUM_FAST                                                                            \
+                   ? launch_mfma_t<T, O, WW, KK, PP, true>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, bpc,   \
+                                                          qscale, s)                                             \
+                   : launch_mfma_t<T, O, WW, KK, PP, false>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu, bpc,  \
+                                                           qscale, s);
     NPMG_QM(64, 7, 4)
     NPMG_QM(64, 8, 4)
     NPMG_QM(64, 16, 4)

@@ -1,7 +1,7 @@
 // k_query_mfma_inst.hip -- instantiations of the fused MFMA query kernel,
-// built once per (operand type, lobe family) with
-//   -DNPMG_INST_{bf16,f16,fp8} -DNPMG_INST_{vmf,asg}
-// so that the six objects compile in parallel.
+// built once per (operand type, lobe family, numerics) with
+//   -DNPMG_INST_{bf16,f16,fp8} -DNPMG_INST_{vmf,asg} -DNPMG_INST_{exact,fast}
+// so that the twelve objects compile in parallel.
 #include "query_mfma.h"
 
 namespace npmg {
@@ -16,13 +16,20 @@ namespace npmg {
 #error "operand type: define NPMG_INST_bf16, NPMG_INST_f16 or NPMG_INST_fp8"
 #endif
 
+#if defined(NPMG_INST_fast)
+#define NPMG_FAST true
+#elif defined(NPMG_INST_exact)
+#define NPMG_FAST false
+#else
+#error "numerics: define NPMG_INST_exact or NPMG_INST_fast"
+#endif
+
 #define NPMG_INST(WW, KK, PP)                                                                                      \
-    template hipError_t launch_mfma_t<NPMG_TYPES, WW, KK, PP>(                                                     \
+    template hipError_t launch_mfma_t<NPMG_TYPES, WW, KK, PP, NPMG_FAST>(                                          \
         const DevCfg&, const void*, const void*, const void*, const QueryIO&, uint32_t, uint32_t, uint32_t, int, \
         int, const float*, hipStream_t);                                                                           \
-    template hipError_t launch_mfma_queues_t<NPMG_TYPES, WW, KK, PP>(const DevCfg&, const void*, const void*,      \
-                                                                     const QueueSet&, int, int, const float*,      \
-                                                                     hipStream_t);
+    template hipError_t launch_mfma_queues_t<NPMG_TYPES, WW, KK, PP, NPMG_FAST>(                                   \
+        const DevCfg&, const void*, const void*, const QueueSet&, int, int, const float*, hipStream_t);
 
 #if defined(NPMG_INST_vmf)
 NPMG_INST(64, 7, 4)

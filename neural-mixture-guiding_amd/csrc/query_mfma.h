@@ -13,6 +13,8 @@
 // (normaliser, softmax, log-sum-exp and sampling stay FP32).
 #pragma once
 
+#include <type_traits>
+
 #include "asg.h"
 #include "device.h"
 #include "kernels.h"
@@ -32,7 +34,7 @@ using namespace dev;
 // qscale holds the per-layer power-of-two multipliers: [0] input features ->
 // E4M3, [l] accumulators of layer l - 1 -> E4M3 inputs of layer l, [n_layers]
 // output accumulators -> raw outputs.
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, const T* __restrict__ table,
                                                 const T* __restrict__ genc, const QueryIO& io, uint32_t n,
                                                 uint32_t q0, uint32_t q_end, int lane,
@@ -47,6 +49,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
     const int h = lane >> 5, j = lane & 31;
     const int KB0 = c.enc_padded / 16;
     const uint32_t qa = min(q0 + (uint32_t)j, q_end - 1u), qb = min(q0 + 32u + (uint32_t)j, q_end - 1u);
+    constexpr bool fast = FAST;  // fast OneBlob bins and mixture ("numerics": "fast")
     float ina[16], inb[16];
 #pragma unroll
     for (int d = 0; d < 16; ++d) {
@@ -59,8 +62,8 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
     // Layer 0: encode chunk by chunk straight into B fragments.
     for (int kb = 0; kb < KB0; ++kb) {
         float va[8], vb[8];
-        encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
-        encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
+        encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n, fast);
+        encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n, fast);
         if constexpr (F8) {
             const float s0 = qscale[0];
 #pragma unroll
@@ -158,15 +161,18 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
             for (int o = 0; o < NOUT; ++o) io.raw_out[(size_t)o * n + q] = out[o];
         }
         if constexpr (P == 6) {
-            if (io.u || io.n_eval > 0) asg_query<K>(c, [&](int jj) { return out[jj]; }, io, q, n);
+            if (io.u || io.n_eval > 0)
+                asg_query_m<K, std::conditional_t<FAST, FastMath, ExactMath>>([&](int jj) { return out[jj]; }, io, q,
+                                                                               n);
         } else if (io.u || io.n_eval > 0 || io.out_sel) {
-            vmf_query_cfg<K, false>(c, [&](int jj) { return out[jj]; }, io, q, n);
+            vmf_query<K, std::conditional_t<FAST, FastMath, ExactMath>, false>(c, [&](int jj) { return out[jj]; }, io,
+                                                                                q, n);
         }
     }
 }
 
 // Queries [q_begin, q_end) of SoA arrays with row stride n.
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const O* __restrict__ wfrag,
                                                     const T* __restrict__ table, const T* __restrict__ genc,
                                                     const QueryIO io, uint32_t n, uint32_t n_frag, uint32_t q_begin,
@@ -180,13 +186,13 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const O* __r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t n_tiles = (q_end - q_begin + 63u) / 64u;
     for (uint32_t tile = blockIdx.x * 4u + (uint32_t)wave; tile < n_tiles; tile += gridDim.x * 4u)
-        query_tile_mfma<T, O, W, K, P>(c, sw, table, genc, io, n, q_begin + tile * 64u, q_end, lane, qscale);
+        query_tile_mfma<T, O, W, K, P, FAST>(c, sw, table, genc, io, n, q_begin + tile * 64u, q_end, lane, qscale);
 }
 
 // Several queues in one persistent launch: each wave pulls 64-query tiles
 // from a device counter over the concatenated tiles of the queues (counts on
 // the device) until all are taken.
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 __global__ void __launch_bounds__(256) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
                                                            const T* __restrict__ table, const QueueSet qs,
                                                            uint32_t n_frag, const float* __restrict__ qscale) {
@@ -213,21 +219,21 @@ __global__ void __launch_bounds__(256) k_query_mfma_queues(const DevCfg c, const
                 qi = i;
                 base = first[i];
             }
-        query_tile_mfma<T, O, W, K, P>(c, sw, table, static_cast<const T*>(qs.genc[qi]), qs.io[qi], qs.stride[qi],
+        query_tile_mfma<T, O, W, K, P, FAST>(c, sw, table, static_cast<const T*>(qs.genc[qi]), qs.io[qi], qs.stride[qi],
                                        (w - base) * 64u, min(qs.counts[qi], qs.stride[qi]), lane, qscale);
     }
 }
 
 }  // namespace
 
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const QueryIO& io,
                          uint32_t n, uint32_t q_begin, uint32_t q_end, int n_cu, int bpc, const float* qscale,
                          hipStream_t s) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const uint32_t n_frag = n_mlp / 512u;
     const size_t lds = (size_t)n_frag * 64u * sizeof(typename mf::Frag<O>::vec);
-    auto kern = k_query_mfma<T, O, W, K, P>;
+    auto kern = k_query_mfma<T, O, W, K, P, FAST>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
@@ -243,13 +249,13 @@ hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, 
     return hipGetLastError();
 }
 
-template <class T, class O, int W, int K, int P>
+template <class T, class O, int W, int K, int P, bool FAST>
 hipError_t launch_mfma_queues_t(const DevCfg& c, const void* wfrag, const void* table, const QueueSet& qs, int n_cu,
                                 int blocks_per_cu, const float* qscale, hipStream_t s) {
     const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
     const uint32_t n_frag = n_mlp / 512u;
     const size_t lds = (size_t)n_frag * 64u * sizeof(typename mf::Frag<O>::vec);
-    auto kern = k_query_mfma_queues<T, O, W, K, P>;
+    auto kern = k_query_mfma_queues<T, O, W, K, P, FAST>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
