@@ -36,26 +36,17 @@ BYTES_PER_QUERY = 36 + 12 + 12 + 48 + 12 + 4 + 16 + 4
 GRID_FEATURES = 16 * 2  # levels x features of the full configuration
 
 
-def profiled_traffic(precision):
-    """HBM bytes per launch of the fused query kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+def profiled(precision, key, queries_per_launch):
+    """A per-launch figure of the fused query kernel from the committed rocprofv3
+    counter summary: "<key>_per_query" scaled to the launch, or "<key>_per_launch"
+    as written; None when the summary lacks it."""
     path = os.path.join(ROOT, "profiles", f"k_query_{precision}_full_counters.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
-def profiled_valu(precision):
-    """VALU wave-instructions per launch of the fused query kernel from the
-    committed rocprofv3 SQ counter summary, or None."""
-    path = os.path.join(ROOT, "profiles", f"k_query_{precision}_full_counters.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("valu_wave_instructions_per_launch:"):
+        if line.startswith(key + "_per_query:"):
+            return float(line.split(":", 1)[1]) * queries_per_launch
+        if line.startswith(key + "_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -161,6 +152,8 @@ def main():
 
     def measure_queries(precision, numerics="fast"):
         net = npmg.GuidingNetwork(full_cfg(precision, numerics), seed=7272)
+        # launch shape from the library's autotuner (pipeline chunks x workgroups per CU), untimed
+        tuned = net.autotune(N_QUERIES) if precision != "fp32" else net.tuning()
 
         def query_step():
             st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
@@ -174,7 +167,8 @@ def main():
             log(f"{precision} ({numerics} numerics) infer+sample+pdf: {ms:.3f} ms per 2^21 queries")
             net.close()
             return None, ms
-        log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s)")
+        log(f"{precision} infer+sample+pdf: {ms:.3f} ms per 2^21 queries ({N_QUERIES / ms / 1e3:.1f} M queries/s), "
+            f"tuning {tuned}")
         # per-kernel launch durations: HIP events recorded by the library around each
         # launch, on the stream that launch runs on (a separate pass, not the timed one)
         net.timing(True)
@@ -186,6 +180,7 @@ def main():
             if cnt:
                 kern[name] = {"avg_ms_per_launch": tot / cnt, "launches_per_step": cnt / args.steps}
         net.timing(False)
+        kern["tuning"] = {"pipeline_chunks": tuned[0], "blocks_per_cu": tuned[1]}
         per_kernel[precision] = kern
         return net, ms
 
@@ -334,7 +329,7 @@ def main():
     alg_bytes = (BYTES_PER_QUERY + genc_bytes) * q_per_launch + table_bytes
     achieved = alg_bytes / (kq["avg_ms_per_launch"] * 1e-3) / 1e9
     flops = 2 * net.n_mlp * q_per_launch
-    traffic = profiled_traffic(args.precision)
+    traffic = profiled(args.precision, "traffic_bytes", q_per_launch)  # FETCH_SIZE + WRITE_SIZE, calibrated
     result = {
         "metric": METRIC,
         "value": qps,
@@ -368,7 +363,7 @@ def main():
             "peak_per_s": VALU_ISSUE_PEAK,
             "frac": v / (kq["avg_ms_per_launch"] * 1e-3) / VALU_ISSUE_PEAK,
             "source": "profiles/k_query_%s_full_counters.md (SQ_INSTS_VALU)" % args.precision})(
-            profiled_valu(args.precision)),
+            profiled(args.precision, "valu_wave_instructions", q_per_launch)),
         "mlp_compute": {"flops_per_launch": flops,
                         "achieved_tflops": flops / (kq["avg_ms_per_launch"] * 1e-3) / 1e12,
                         "peak_tflops": MFMA_BF16_PEAK_TFLOPS if half else FP32_PEAK_TFLOPS},

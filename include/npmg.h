@@ -169,6 +169,12 @@ npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream);
 /* The launch shape in use: pipeline chunks and workgroups per CU (0 = as
  * many as fit). */
 npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_per_cu);
+/* Sets the launch shape without timing, e.g. one that npmg_tuning reported
+ * in an earlier process: chunks 1..8 (used by the BF16/FP16 path on batches
+ * of 2^18 queries or more), blocks_per_cu 0..64 (0 = as many as fit; larger
+ * values are capped at what fits). Changes no result bit. No reference
+ * counterpart (the reference has no launch-shape tuning). */
+npmg_status npmg_set_tuning(npmg_ctx* ctx, uint32_t chunks, uint32_t blocks_per_cu);
 
 /* FP8 inference ("inference_precision": "fp8" with "precision": "bf16"): the
  * query path's MLP runs on E4M3 weights and activations with FP32

@@ -1282,6 +1282,15 @@ npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_sc
     });
 }
 
+npmg_status npmg_set_tuning(npmg_ctx* ctx, uint32_t chunks, uint32_t blocks_per_cu) {
+    return guarded(ctx, [&] {
+        need(chunks >= 1u && chunks <= (uint32_t)npmg_ctx::kPipe, "pipeline chunks must be 1..8");
+        need(blocks_per_cu <= 64u, "workgroups per CU must be 0..64");
+        ctx->tune_chunks = chunks;
+        ctx->tune_bpc = (int)blocks_per_cu;
+    });
+}
+
 npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_per_cu) {
     if (!ctx || !chunks || !blocks_per_cu) return NPMG_EINVAL;
     *chunks = ctx->half() && mfma_split_grid(ctx->cfg.d) ? ctx->tune_chunks : 1u;

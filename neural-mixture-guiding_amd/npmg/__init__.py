@@ -100,6 +100,7 @@ SIGNATURES = {
                                   ctypes.POINTER(_u32), _vp]),
     "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "npmg_set_tuning": (_int, [_vp, _u32, _u32]),
 }
 
 
@@ -541,6 +542,10 @@ class GuidingNetwork:
         c, b = _u32(), _u32()
         self._check(self._lib.npmg_tuning(self._h, ctypes.byref(c), ctypes.byref(b)))
         return c.value, b.value
+
+    def set_tuning(self, chunks: int, blocks_per_cu: int):
+        """Sets the query path's launch shape (e.g. a saved autotune result)."""
+        self._check(self._lib.npmg_set_tuning(self._h, int(chunks), int(blocks_per_cu)))
 
     def train_step_graph(self, inputs, tgt_dir, tgt_rgb, tgt_pdf, loss_out):
         """train_step as one captured HIP graph (replayed while the buffers are the same)."""

@@ -32,3 +32,28 @@ class NetworkFunction(torch.autograd.Function):
 def network_apply(net, inputs, params):
     """raw outputs [n_out][n] = net(inputs; params), differentiable in params."""
     return NetworkFunction.apply(net, inputs, params)
+
+
+class MixtureLossFunction(torch.autograd.Function):
+    """Mean KL loss -f/q log(p + eps) of the mixture over a batch
+    (compute_dL_doutput_divergence, train.cu:69-124) with its hand-written
+    adjoint as the backward: dL/draw from npmg_mixture_adjoint with loss
+    scale 1 (so already divided by the batch size). The adjoint zeroes
+    non-finite gradients and applies the configuration's upper clamp, as the
+    reference does; set "loss": {"grad_clamp": 0} for an unclamped gradient."""
+
+    @staticmethod
+    def forward(ctx, net, raw, tgt_dir, tgt_rgb, tgt_pdf):
+        dout, loss = net.mixture_adjoint(raw.detach().contiguous(), tgt_dir, tgt_rgb, tgt_pdf, 1.0)
+        ctx.save_for_backward(dout)
+        return loss.mean()
+
+    @staticmethod
+    def backward(ctx, grad):
+        (dout,) = ctx.saved_tensors
+        return None, dout * grad, None, None, None
+
+
+def mixture_loss(net, raw, tgt_dir, tgt_rgb, tgt_pdf):
+    """Mean mixture KL loss of raw outputs [n_out][n], differentiable in raw."""
+    return MixtureLossFunction.apply(net, raw, tgt_dir, tgt_rgb, tgt_pdf)

@@ -132,4 +132,15 @@ def test_autotune_keeps_results(gpu, prec, mix):
     torch.cuda.synchronize()
     for k in before:
         assert torch.equal(before[k], after[k]), k
+    # shapes set by hand: every one gives the same bits; out-of-range ones are rejected
+    for shape in ((1, 0), (8, 1), (3, 2), (2, 64)):
+        net.set_tuning(*shape)
+        assert net.tuning() == shape
+        again = net.sample_pdf(q["inputs"], nrm, q["u"], q["eval_dirs"], cosine_aware=cosine)
+        for k in before:
+            assert torch.equal(before[k], again[k]), (shape, k)
+    for bad in ((0, 0), (9, 0), (4, 65)):
+        with pytest.raises(npmg.NpmgError):
+            net.set_tuning(*bad)
+    assert net.tuning() == (2, 64)
     net.close()
