@@ -146,6 +146,13 @@ Config parse_config(const std::string& text) {
     d.enc_width = off;
     d.enc_padded = (off + 15) / 16 * 16;
     if (d.grid_enc >= 0) grid_layout(d, log2_T, base_res, pls);
+    d.ob_kb0 = d.ob_nkb = d.ob_in = 0;
+    if (d.n_enc == 2 && d.grid_enc == 0 && (d.n_levels * d.F) % 16 == 0 && d.enc_type[1] == ENC_ONEBLOB &&
+        d.enc_param[1] == 16 && d.enc_width == d.enc_padded) {
+        d.ob_kb0 = d.n_levels * d.F / 16;
+        d.ob_nkb = d.enc_ndims[1];
+        d.ob_in = d.enc_begin[1];
+    }
 
     // ---- mixture ----
     const Json empty;

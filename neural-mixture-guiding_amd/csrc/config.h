@@ -31,6 +31,12 @@ struct DevCfg {
     int enc_type[kMaxEnc], enc_begin[kMaxEnc], enc_ndims[kMaxEnc], enc_param[kMaxEnc], enc_offset[kMaxEnc];
     int enc_width, enc_padded;
     int grid_enc;  // index of the grid encoding, -1 if none
+    // Layer-0 layout "grid, then OneBlob(16)": when ob_nkb > 0, the input
+    // features are the grid's (K-blocks [0, ob_kb0) of 16 features) followed by
+    // 16 OneBlob bins of each input dim ob_in .. ob_in + ob_nkb - 1 (one
+    // K-block per dim) and nothing else, so the MFMA query kernels can encode
+    // them without the general per-feature dispatch.
+    int ob_kb0, ob_nkb, ob_in;
     int grid_type, hash_type, n_levels, F, interp;
     uint32_t offsets[kMaxLevels + 1];
     float scales[kMaxLevels];

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "config.h"
 #include "kernels.h"
 #include "npmg_math.h"
@@ -48,8 +50,9 @@ __device__ __forceinline__ V3 normalize(V3 a) { return divs(a, length(a)); }
 // FP32 instantiation. FastMath ("numerics": "fast") uses the hardware
 // transcendentals (v_exp_f32 / v_log_f32 base 2, v_sin_f32 / v_cos_f32 in
 // revolutions, v_rcp_f32, v_sqrt_f32: about 1 ulp each) for exp, log,
-// sin/cos, division and sqrt; expm1 and log1p stay exact (their arguments
-// can be small).
+// sin/cos, division and sqrt; expm1 is a Taylor polynomial below |x| = 1/2
+// (relative error below 3e-7) and the hardware exponential minus 1 beyond;
+// log1p stays exact.
 struct ExactMath {
     static __device__ __forceinline__ float exp(float x) { return fm::exp(x); }
     static __device__ __forceinline__ float log(float x) { return fm::log(x); }
@@ -62,7 +65,16 @@ struct ExactMath {
 struct FastMath {
     static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
     static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
-    static __device__ __forceinline__ float expm1(float x) { return fm::expm1(x); }
+    static __device__ __forceinline__ float expm1(float x) {
+        float p = __builtin_fmaf(x, 1.0f / 5040.0f, 1.0f / 720.0f);
+        p = __builtin_fmaf(p, x, 1.0f / 120.0f);
+        p = __builtin_fmaf(p, x, 1.0f / 24.0f);
+        p = __builtin_fmaf(p, x, 1.0f / 6.0f);
+        p = __builtin_fmaf(p, x, 0.5f);
+        p = __builtin_fmaf(p, x, 1.0f);
+        const float e = __builtin_amdgcn_exp2f(x * 1.44269504088896341f) - 1.0f;
+        return __builtin_fabsf(x) < 0.5f ? p * x : e;
+    }
     static __device__ __forceinline__ float log1p(float x) { return fm::log1p(x); }
     static __device__ __forceinline__ void sincos(float x, float* s, float* c) {
         const float r = x * 0.159154943091895336f;  // revolutions
@@ -218,6 +230,31 @@ __device__ __forceinline__ int level_corners(const DevCfg& c, int l, float x0, f
     return 8;
 }
 
+// The F features of one table entry. An entry is F * sizeof(T) bytes at a
+// multiple of its size from the table start, so when that size is 4, 8 or 16
+// bytes and the table start is aligned to it (`vec`), one vector load fetches
+// the entry: one gather per corner instead of one per feature.
+template <class T, int F>
+__device__ __forceinline__ void load_entry(const T* __restrict__ tl, uint32_t idx, bool vec, T (&v)[F]) {
+    constexpr int B = F * (int)sizeof(T);
+    if constexpr (B == 4 || B == 8 || B == 16) {
+        using U = std::conditional_t<B == 4, uint32_t, std::conditional_t<B == 8, uint2, uint4>>;
+        if (vec) {
+            const U u = *reinterpret_cast<const U*>(tl + (size_t)idx * F);
+            __builtin_memcpy(&v[0], &u, B);
+            return;
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) v[f] = tl[(size_t)idx * F + f];
+}
+
+// Whether load_entry may use one vector load on this table (a uniform test).
+template <class T, int F>
+__device__ __forceinline__ bool entry_aligned(const T* table) {
+    return (reinterpret_cast<uintptr_t>(table) & (uintptr_t)(F * sizeof(T) - 1)) == 0;
+}
+
 // tiny-cuda-nn common_device.h:533-548. The kernel's CDF is monotonic and
 // reaches its clamps; beyond |u| >= 1.0625 the clamped result is exactly 0 or
 // 1, so those evaluations are skipped (bit-identical).
@@ -287,16 +324,14 @@ __device__ __forceinline__ void encode_query(const DevCfg& c, const T* __restric
                         a0 = (float)tl[idx[0] * 2u];
                         a1 = (float)tl[idx[0] * 2u + 1u];
                     } else {
-                        T v[16];
+                        T v[8][2];
+                        const bool vec = entry_aligned<T, 2>(table);
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) load_entry<T, 2>(tl, idx[k], vec, v[k]);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            v[2 * k] = tl[idx[k] * 2u];
-                            v[2 * k + 1] = tl[idx[k] * 2u + 1u];
-                        }
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            a0 = fm::fma_(w[k], (float)v[2 * k], a0);
-                            a1 = fm::fma_(w[k], (float)v[2 * k + 1], a1);
+                            a0 = fm::fma_(w[k], (float)v[k][0], a0);
+                            a1 = fm::fma_(w[k], (float)v[k][1], a1);
                         }
                     }
                     feed(a0);
@@ -411,10 +446,16 @@ template <int K, class MT = ExactMath>
 __device__ __forceinline__ void apply_cosine(const DevCfg& c, V3 nrm, Mixture<K>& m, bool want_angles,
                                              float (&theta_out)[K], float (&phi_out)[K]) {
     const bool ref = c.numerics == NUM_REFERENCE;
-    float nt, np;
-    cart_to_sph(nrm, &nt, &np);
     const float kb = c.cos_kappa;
-    const V3 mu_b = sph_to_cart<MT>(nt, np);
+    V3 mu_b;
+    if constexpr (__is_same(MT, ExactMath)) {
+        // the reference's round trip of the normal through (theta, phi)
+        float nt, np;
+        cart_to_sph(nrm, &nt, &np);
+        mu_b = sph_to_cart<MT>(nt, np);
+    } else {
+        mu_b = normalize_m<MT>(nrm);
+    }
     const float norm_b = ref ? ref_norm(kb, 1e-3f) : stable_norm<MT>(kb);
     m.total = 0.0f;
 #pragma unroll
@@ -732,16 +773,17 @@ __device__ __forceinline__ void grid_level_features(const DevCfg& c, const T* __
     float w[8];
     const int nc = level_corners(c, l, x0, x1, x2, idx, w);
     const T* tl = table + (size_t)c.offsets[l] * F;
+    const bool vec = entry_aligned<T, F>(table);
     if (nc == 1) {
+        T v0[F];
+        load_entry<T, F>(tl, idx[0], vec, v0);
 #pragma unroll
-        for (int f = 0; f < F; ++f) out[f] = (float)tl[idx[0] * (uint32_t)F + f];
+        for (int f = 0; f < F; ++f) out[f] = (float)v0[f];
         return;
     }
     T v[8][F];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int f = 0; f < F; ++f) v[k][f] = tl[idx[k] * (uint32_t)F + f];
+    for (int k = 0; k < 8; ++k) load_entry<T, F>(tl, idx[k], vec, v[k]);
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         float a = 0.0f;

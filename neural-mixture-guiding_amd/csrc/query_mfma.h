@@ -26,6 +26,82 @@ namespace {
 using namespace dev;
 
 
+// Two 16-bit operands in one register, `lo` in bits 0..15 (the element order of
+// a fragment), each rounded to nearest even like mf::pack8.
+template <class T>
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    const T a = (T)lo, b = (T)hi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+// Bins 8h .. 8h+7 of a 16-bin OneBlob encoding of x, as one B fragment (fast
+// numerics). The quartic kernel of radius one bin (tiny-cuda-nn
+// common_device.h:533-548), centred at s = 16 x = kx + t with t in [0, 1),
+// puts all its mass into bins kx - 1, kx and kx + 1, wrapped mod 16 like
+// kernel_one_blob_soa (oneblob.h:114-123): 1/2 - S(t), S(t) + S(1 - t) and
+// 1/2 - S(1 - t), with S(u) = 15/16 (u - 2u^3/3 + u^5/5) the odd part of the
+// kernel's CDF. Bins b, b + 1 are the 32-bit window at 16-bit slot
+// e = (b - kx + 2) mod 16 of the slot sequence (0, m[kx-1], m[kx], m[kx+1], 0, ...).
+template <class T>
+__device__ __forceinline__ typename mf::Frag<T>::vec oneblob16_frag(float x, int h) {
+    const float s = x * 16.0f;
+    const float fl = __builtin_floorf(s);
+    const float t = s - fl;
+    const int kx = (int)fl;
+    auto odd = [](float u) {
+        const float u2 = u * u;
+        return u * __builtin_fmaf(u2, __builtin_fmaf(u2, 0.1875f, -0.625f), 0.9375f);
+    };
+    const float st = odd(t), sr = odd(1.0f - t);
+    const uint64_t slots = ((uint64_t)pack2<T>(st + sr, 0.5f - sr) << 32) | (uint64_t)pack2<T>(0.0f, 0.5f - st);
+    const int base = 8 * h + 2 - kx;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 d;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t e = (uint32_t)(base + 2 * j) & 15u;
+        d[j] = e < 4u ? (uint32_t)(slots >> (16u * e)) : 0u;
+    }
+    return __builtin_bit_cast(typename mf::Frag<T>::vec, d);
+}
+
+// Layer 0 of the "grid, then OneBlob(16)" layout (DevCfg::ob_nkb > 0) for a
+// 16-bit operand type: the grid K-blocks straight from the level-major
+// features, then one K-block of oneblob16_frag bins per OneBlob input dim.
+template <class T, int TH>
+__device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, const T* __restrict__ genc,
+                                                 const QueryIO& io, uint32_t n, uint32_t qa, uint32_t qb, int lane,
+                                                 mf::f32x16 (&acc)[TH][2]) {
+    using V = typename mf::Frag<T>::vec;
+    const int h = lane >> 5, KB0 = c.enc_padded / 16;
+    for (int kb = 0; kb < c.ob_kb0; ++kb) {
+        V ba, bb;
+        const T* src = genc + (size_t)(16 * kb + 8 * h) * n;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            ba[i] = src[(size_t)i * n + qa];
+            bb[i] = src[(size_t)i * n + qb];
+        }
+#pragma unroll
+        for (int t = 0; t < TH; ++t) {
+            const V a = mf::load_frag<T>(sw, t * KB0 + kb, lane);
+            acc[t][0] = mf::Frag<T>::mma(a, ba, acc[t][0]);
+            acc[t][1] = mf::Frag<T>::mma(a, bb, acc[t][1]);
+        }
+    }
+    for (int d = 0; d < c.ob_nkb; ++d) {
+        const float* row = io.in + (size_t)(c.ob_in + d) * n;
+        const V ba = oneblob16_frag<T>(row[qa], h), bb = oneblob16_frag<T>(row[qb], h);
+        const int kb = c.ob_kb0 + d;
+#pragma unroll
+        for (int t = 0; t < TH; ++t) {
+            const V a = mf::load_frag<T>(sw, t * KB0 + kb, lane);
+            acc[t][0] = mf::Frag<T>::mma(a, ba, acc[t][0]);
+            acc[t][1] = mf::Frag<T>::mma(a, bb, acc[t][1]);
+        }
+    }
+}
+
 // Queries [q_begin, q_end) of SoA arrays with row stride n.
 // One 64-query tile [q0, min(q0 + 64, q_end)) of a query stream with row stride n:
 // encode, MLP on MFMA (weights in LDS at sw), mixture, sample and pdfs.
@@ -50,34 +126,43 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
     const int KB0 = c.enc_padded / 16;
     const uint32_t qa = min(q0 + (uint32_t)j, q_end - 1u), qb = min(q0 + 32u + (uint32_t)j, q_end - 1u);
     constexpr bool fast = FAST;  // fast OneBlob bins and mixture ("numerics": "fast")
-    float ina[16], inb[16];
-#pragma unroll
-    for (int d = 0; d < 16; ++d) {
-        ina[d] = d < c.n_in ? io.in[(size_t)d * n + qa] : 0.0f;
-        inb[d] = d < c.n_in ? io.in[(size_t)d * n + qb] : 0.0f;
-    }
     mf::f32x16 acc[TH][2];
 #pragma unroll
     for (int t = 0; t < TH; ++t) acc[t][0] = acc[t][1] = mf::f32x16{};
-    // Layer 0: encode chunk by chunk straight into B fragments.
-    for (int kb = 0; kb < KB0; ++kb) {
-        float va[8], vb[8];
-        encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n, fast);
-        encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n, fast);
-        if constexpr (F8) {
-            const float s0 = qscale[0];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                va[e] = __builtin_amdgcn_fmed3f(va[e] * s0, -mf::kFp8Max, mf::kFp8Max);
-                vb[e] = __builtin_amdgcn_fmed3f(vb[e] * s0, -mf::kFp8Max, mf::kFp8Max);
-            }
+    bool layer0_done = false;
+    if constexpr (!F8 && FAST) {
+        if (c.ob_nkb > 0 && genc) {
+            layer0_oneblob16<O, TH>(c, sw, genc, io, n, qa, qb, lane, acc);
+            layer0_done = true;
         }
-        const V ba = mf::pack8<O>(va), bb = mf::pack8<O>(vb);
+    }
+    if (!layer0_done) {
+        float ina[16], inb[16];
 #pragma unroll
-        for (int t = 0; t < TH; ++t) {
-            const V a = mf::load_frag<O>(sw, t * KB0 + kb, lane);
-            acc[t][0] = mf::Frag<O>::mma(a, ba, acc[t][0]);
-            acc[t][1] = mf::Frag<O>::mma(a, bb, acc[t][1]);
+        for (int d = 0; d < 16; ++d) {
+            ina[d] = d < c.n_in ? io.in[(size_t)d * n + qa] : 0.0f;
+            inb[d] = d < c.n_in ? io.in[(size_t)d * n + qb] : 0.0f;
+        }
+        // Layer 0: encode chunk by chunk straight into B fragments.
+        for (int kb = 0; kb < KB0; ++kb) {
+            float va[8], vb[8];
+            encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n, fast);
+            encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n, fast);
+            if constexpr (F8) {
+                const float s0 = qscale[0];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    va[e] = __builtin_amdgcn_fmed3f(va[e] * s0, -mf::kFp8Max, mf::kFp8Max);
+                    vb[e] = __builtin_amdgcn_fmed3f(vb[e] * s0, -mf::kFp8Max, mf::kFp8Max);
+                }
+            }
+            const V ba = mf::pack8<O>(va), bb = mf::pack8<O>(vb);
+#pragma unroll
+            for (int t = 0; t < TH; ++t) {
+                const V a = mf::load_frag<O>(sw, t * KB0 + kb, lane);
+                acc[t][0] = mf::Frag<O>::mma(a, ba, acc[t][0]);
+                acc[t][1] = mf::Frag<O>::mma(a, bb, acc[t][1]);
+            }
         }
     }
     int fb = TH * KB0;
