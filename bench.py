@@ -36,11 +36,12 @@ BYTES_PER_QUERY = 36 + 12 + 12 + 48 + 12 + 4 + 16 + 4
 GRID_FEATURES = 16 * 2  # levels x features of the full configuration
 
 
-def profiled(precision, key, queries_per_launch):
-    """A per-launch figure of the fused query kernel from the committed rocprofv3
+def profiled(precision, key, queries_per_launch, kernel="k_query"):
+    """A per-launch figure of `kernel` (k_query: the fused query kernel;
+    k_grid_encode: the level-major grid encode) from the committed rocprofv3
     counter summary: "<key>_per_query" scaled to the launch, or "<key>_per_launch"
     as written; None when the summary lacks it."""
-    path = os.path.join(ROOT, "profiles", f"k_query_{precision}_full_counters.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{precision}_full_counters.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -108,6 +109,9 @@ def main():
     ap.add_argument("--only", default="all", choices=["all", "query", "frame"],
                     help="query: time only the fused query kernel (for counter runs); "
                          "frame: only the anisotropic 8-bounce frame")
+    ap.add_argument("--tuning", default=None, metavar="CHUNKS,BLOCKS_PER_CU",
+                    help="use this query launch shape instead of autotuning (counter runs: one launch "
+                         "of each kernel per step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy 5 times (HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -153,7 +157,11 @@ def main():
     def measure_queries(precision, numerics="fast"):
         net = npmg.GuidingNetwork(full_cfg(precision, numerics), seed=7272)
         # launch shape from the library's autotuner (pipeline chunks x workgroups per CU), untimed
-        tuned = net.autotune(N_QUERIES) if precision != "fp32" else net.tuning()
+        if args.tuning and precision != "fp32":
+            net.set_tuning(*[int(v) for v in args.tuning.split(",")])
+            tuned = net.tuning()
+        else:
+            tuned = net.autotune(N_QUERIES) if precision != "fp32" else net.tuning()
 
         def query_step():
             st = lib.npmg_sample_pdf(net._h, P(q["inputs"]), P(q["normal"]), P(q["u"]), P(q["eval_dirs"]), 4,
@@ -320,16 +328,43 @@ def main():
     sps = N_TRAIN / (ms_t * 1e-3)
 
     half = args.precision != "fp32"
-    # Roofline of the dominant kernel, the fused query kernel: algorithmic bytes of
-    # one launch (its share of the queries) over its average launch duration.
+    # Per-launch figures of the two kernels of the step: algorithmic bytes of one
+    # launch (its share of the queries) over its average launch duration.
     kq = per_kernel[args.precision]["query"]
     q_per_launch = N_QUERIES / kq["launches_per_step"]
     genc_bytes = (GRID_FEATURES * 2) if half else 0  # level-major grid features read back (BF16)
     table_bytes = 0 if half else net.n_grid * 4       # FP32 path gathers the table itself
-    alg_bytes = (BYTES_PER_QUERY + genc_bytes) * q_per_launch + table_bytes
+    pos_bytes = 12 if half else 0                     # MFMA path: the positions are the encode's input
+    alg_bytes = (BYTES_PER_QUERY - pos_bytes + genc_bytes) * q_per_launch + table_bytes
     achieved = alg_bytes / (kq["avg_ms_per_launch"] * 1e-3) / 1e9
     flops = 2 * net.n_mlp * q_per_launch
     traffic = profiled(args.precision, "traffic_bytes", q_per_launch)  # FETCH_SIZE + WRITE_SIZE, calibrated
+    fused_roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                      "kernel": ("k_query_mfma" if half else "k_query_fp32") +
+                                " (fused OneBlob encode + MLP + mixture + sample + pdf)",
+                      "algorithmic_bytes_per_launch": alg_bytes,
+                      "avg_launch_ms": kq["avg_ms_per_launch"], "launches_per_step": kq["launches_per_step"]}
+    roofline = fused_roofline
+    ke = per_kernel[args.precision].get("grid_encode")
+    encode_roofline = None
+    if ke:
+        # level-major grid encode: positions (12 B) in and L F BF16 features (64 B) out per
+        # query, the BF16 table (26 MB) read once
+        qe = N_QUERIES / ke["launches_per_step"]
+        e_bytes = (12 + GRID_FEATURES * 2) * qe + net.n_grid * 2
+        e_ach = e_bytes / (ke["avg_ms_per_launch"] * 1e-3) / 1e9
+        encode_roofline = {"bound": "hbm", "achieved": e_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": e_ach / HBM_PEAK_GBS,
+                           "traffic": profiled(args.precision, "traffic_bytes", qe, "k_grid_encode"),
+                           "kernel": "k_grid_encode (level-major hash-grid gathers, 16 levels x 8 corners)",
+                           "algorithmic_bytes_per_launch": e_bytes,
+                           "avg_launch_ms": ke["avg_ms_per_launch"], "launches_per_step": ke["launches_per_step"],
+                           "limiter": "L1 cache-line lookups of random gathers (about 5 per query and level), "
+                                      "not HBM bytes",
+                           "l1_accesses_per_launch": profiled(args.precision, "l1_accesses", qe, "k_grid_encode")}
+        if ke["avg_ms_per_launch"] * ke["launches_per_step"] > kq["avg_ms_per_launch"] * kq["launches_per_step"]:
+            roofline = encode_roofline  # the dominant kernel of the step
     result = {
         "metric": METRIC,
         "value": qps,
@@ -348,12 +383,9 @@ def main():
                    "queries_per_step": N_QUERIES, "train_batch": N_TRAIN, "precision": args.precision},
         "train_samples_per_s": sps,
         "train_ms_per_step": ms_t,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_query_mfma" if half else "k_query_fp32") +
-                               " (fused OneBlob encode + MLP + mixture + sample + pdf)",
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": kq["avg_ms_per_launch"], "launches_per_step": kq["launches_per_step"]},
+        "roofline": roofline,
+        "fused_kernel_roofline": fused_roofline,
+        "grid_encode_roofline": encode_roofline,
         "kernel_launches": per_kernel,
         # the kernel's actual limiter: VALU issue (4 cycles per 64-lane FP32 instruction per SIMD,
         # 1024 SIMDs at 2.4 GHz = 614.4 G wave-instructions/s; transcendentals issue at quarter rate)

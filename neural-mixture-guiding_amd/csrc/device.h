@@ -255,6 +255,56 @@ __device__ __forceinline__ bool entry_aligned(const T* table) {
     return (reinterpret_cast<uintptr_t>(table) & (uintptr_t)(F * sizeof(T) - 1)) == 0;
 }
 
+// The entries of the 8 corners of one level (corner k at idx[k]; bit 0 of k
+// is the x offset). Corners 2m and 2m + 1 differ only in x, and their entries
+// usually share an aligned 16-byte group: dense rows put x + 1 at idx + 1, and
+// the coherent-prime hash (x * 1 ^ ...) keeps x + 1 within the group unless
+// the increment carries past its low bits. So when an entry is 4 bytes and
+// the table start is 16-byte aligned (`g16`; level offsets are multiples
+// of 8 entries), one 16-byte load of corner 2m's group also yields corner
+// 2m + 1 for most lanes; the others load it on their own. Each gather costs a
+// cache-line lookup, so this saves about a third of them (same values, same
+// order of accumulation afterwards). Only for 4-byte entries (16-bit tables,
+// F = 2): with FP32 entries the gathers of the FP32 paths, which read every
+// level of a 52 MB table per query, are bound by the bytes they move, and a
+// 16-byte group doubles those (measured slower).
+template <class T, int F>
+__device__ __forceinline__ void gather_corners(const T* __restrict__ tl, const uint32_t (&idx)[8], bool g16, bool vec,
+                                               T (&v)[8][F]) {
+    constexpr int B = F * (int)sizeof(T);
+    if constexpr (B == 4) {
+        if (g16) {
+            constexpr uint32_t G = 4u;  // entries per group
+            using E = uint32_t;
+            const E* te = reinterpret_cast<const E*>(tl);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t i0 = idx[2 * m], i1 = idx[2 * m + 1];
+                const uint4 p = *reinterpret_cast<const uint4*>(te + (i0 & ~(G - 1u)));
+                auto pick = [&](uint32_t i) {
+                    const uint32_t lo = (i & 1u) ? p.y : p.x, hi = (i & 1u) ? p.w : p.z;
+                    return (i & 2u) ? hi : lo;
+                };
+                const E e0 = pick(i0);
+                E e1;
+                if ((i0 ^ i1) < G) e1 = pick(i1);
+                else e1 = te[i1];
+                __builtin_memcpy(&v[2 * m][0], &e0, B);
+                __builtin_memcpy(&v[2 * m + 1][0], &e1, B);
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) load_entry<T, F>(tl, idx[k], vec, v[k]);
+}
+
+// Whether gather_corners may use 16-byte group loads on this table (uniform).
+template <class T>
+__device__ __forceinline__ bool group_aligned(const T* table) {
+    return (reinterpret_cast<uintptr_t>(table) & (uintptr_t)15) == 0;
+}
+
 // tiny-cuda-nn common_device.h:533-548. The kernel's CDF is monotonic and
 // reaches its clamps; beyond |u| >= 1.0625 the clamped result is exactly 0 or
 // 1, so those evaluations are skipped (bit-identical).
@@ -325,9 +375,7 @@ __device__ __forceinline__ void encode_query(const DevCfg& c, const T* __restric
                         a1 = (float)tl[idx[0] * 2u + 1u];
                     } else {
                         T v[8][2];
-                        const bool vec = entry_aligned<T, 2>(table);
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) load_entry<T, 2>(tl, idx[k], vec, v[k]);
+                        gather_corners<T, 2>(tl, idx, group_aligned(table), entry_aligned<T, 2>(table), v);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             a0 = fm::fma_(w[k], (float)v[k][0], a0);
@@ -782,8 +830,7 @@ __device__ __forceinline__ void grid_level_features(const DevCfg& c, const T* __
         return;
     }
     T v[8][F];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) load_entry<T, F>(tl, idx[k], vec, v[k]);
+    gather_corners<T, F>(tl, idx, group_aligned(table), vec, v);
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         float a = 0.0f;
