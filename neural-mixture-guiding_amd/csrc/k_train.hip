@@ -237,16 +237,32 @@ __global__ void __launch_bounds__(256) k_wgrad(const DevCfg c, const WgradTiles 
         }
 }
 
+// out[i] = sum over the splits k of partials[k][i], in a fixed order: a
+// workgroup takes 32 columns; thread (g, col) adds the rows of group g
+// (8 contiguous ranges, in row order), then the 8 group sums are added in
+// group order. Coalesced 128-byte row segments and 8 loads in flight per
+// column instead of one thread walking every split.
 __global__ void __launch_bounds__(256) k_reduce_splits(const float* __restrict__ partials, uint32_t n_splits,
                                                        uint32_t len, float* __restrict__ out,
                                                        uint32_t* __restrict__ overflow) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    __shared__ float part[8][32];
+    const uint32_t col = threadIdx.x & 31u, g = threadIdx.x >> 5;
+    const uint32_t i = blockIdx.x * 32u + col;
+    const uint32_t per = (n_splits + 7u) / 8u;
+    const uint32_t k0 = min(n_splits, g * per), k1 = min(n_splits, k0 + per);
     float s = 0.0f;
-    if (i < len) {
-        for (uint32_t k = 0; k < n_splits; ++k) s += partials[(size_t)k * len + i];
-        out[i] = s;
+    if (i < len)
+        for (uint32_t k = k0; k < k1; ++k) s += partials[(size_t)k * len + i];
+    part[g][col] = s;
+    __syncthreads();
+    if (g == 0) {
+        float t = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) t += part[r][col];
+        if (i < len) out[i] = t;
+        const bool finite = !(i < len) || __builtin_isfinite(t);
+        if (overflow && !__all(finite ? 1 : 0) && threadIdx.x == 0) atomicOr(overflow, 1u);
     }
-    if (overflow && !__all(__builtin_isfinite(s) ? 1 : 0) && (threadIdx.x & 63u) == 0) atomicOr(overflow, 1u);
 }
 
 // tiny-cuda-nn optimizers/adam.h:47-119 followed by the EMA of
@@ -455,13 +471,13 @@ hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t len = n_mlp_of(c);
-    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, splits, len, dw, overflow);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 31) / 32), dim3(256), 0, s, partials, splits, len, dw, overflow);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_splits(const float* partials, uint32_t n_splits, uint32_t len, float* out, hipStream_t s,
                                 uint32_t* overflow) {
-    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 255) / 256), dim3(256), 0, s, partials, n_splits, len, out,
+    hipLaunchKernelGGL(k_reduce_splits, dim3((len + 31) / 32), dim3(256), 0, s, partials, n_splits, len, out,
                        overflow);
     return hipGetLastError();
 }

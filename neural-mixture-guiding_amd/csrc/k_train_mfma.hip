@@ -529,8 +529,11 @@ size_t train_mask_words(const DevCfg& c, uint32_t n) {
 namespace {
 // Batch splitting of the MFMA weight gradient: one partial row per wave,
 // 4 waves per block, at least 1024 samples per wave and at most 1024 rows.
+// Partial rows of the weight gradient: one wave per row and layer, about 512
+// samples each (2 waves per SIMD at 2^18 samples; the rows are summed by
+// k_reduce_splits).
 void wgrad_split(uint32_t n, uint32_t* rows, uint32_t* q_per_wave) {
-    uint32_t r = (n + 1023u) / 1024u;
+    uint32_t r = (n + 511u) / 512u;
     r = (r + 3u) / 4u * 4u;
     if (r > 1024u) r = 1024u;
     if (r < 4u) r = 4u;
