@@ -102,6 +102,70 @@ __device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, c
     }
 }
 
+// The same 8 OneBlob(16) bins as float values (bin 8h + i is m[kx - 1],
+// m[kx] or m[kx + 1] when (8h + i - kx + 1) mod 16 is 0, 1 or 2, else 0), for
+// the FP8 operands, which scale and round them like any other feature.
+__device__ __forceinline__ void oneblob16_bins(float x, int h, float (&v)[8]) {
+    const float s = x * 16.0f;
+    const float fl = __builtin_floorf(s);
+    const float t = s - fl;
+    const int kx = (int)fl;
+    auto odd = [](float u) {
+        const float u2 = u * u;
+        return u * __builtin_fmaf(u2, __builtin_fmaf(u2, 0.1875f, -0.625f), 0.9375f);
+    };
+    const float st = odd(t), sr = odd(1.0f - t);
+    const float mp = 0.5f - st, mc = st + sr, mn = 0.5f - sr;
+    const int base = 8 * h + 1 - kx;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t r = (uint32_t)(base + i) & 15u;
+        v[i] = r == 0u ? mp : (r == 1u ? mc : (r == 2u ? mn : 0.0f));
+    }
+}
+
+// Layer 0 of the "grid, then OneBlob(16)" layout with FP8 E4M3 operands:
+// features scaled by qscale0 and clamped to the E4M3 range, as in the general
+// path; the grid K-blocks come from the level-major BF16 features.
+template <class T, int TH>
+__device__ __forceinline__ void layer0_oneblob16_fp8(const DevCfg& c, const mf::Fp8* sw, const T* __restrict__ genc,
+                                                     const QueryIO& io, uint32_t n, uint32_t qa, uint32_t qb, int lane,
+                                                     float qscale0, mf::f32x16 (&acc)[TH][2]) {
+    using V = typename mf::Frag<mf::Fp8>::vec;
+    const int h = lane >> 5, KB0 = c.enc_padded / 16;
+    auto put = [&](float (&va)[8], float (&vb)[8], int kb) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            va[e] = __builtin_amdgcn_fmed3f(va[e] * qscale0, -mf::kFp8Max, mf::kFp8Max);
+            vb[e] = __builtin_amdgcn_fmed3f(vb[e] * qscale0, -mf::kFp8Max, mf::kFp8Max);
+        }
+        const V ba = mf::pack8<mf::Fp8>(va), bb = mf::pack8<mf::Fp8>(vb);
+#pragma unroll
+        for (int t = 0; t < TH; ++t) {
+            const V a = mf::load_frag<mf::Fp8>(sw, t * KB0 + kb, lane);
+            acc[t][0] = mf::Frag<mf::Fp8>::mma(a, ba, acc[t][0]);
+            acc[t][1] = mf::Frag<mf::Fp8>::mma(a, bb, acc[t][1]);
+        }
+    };
+    for (int kb = 0; kb < c.ob_kb0; ++kb) {
+        float va[8], vb[8];
+        const T* src = genc + (size_t)(16 * kb + 8 * h) * n;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            va[i] = (float)src[(size_t)i * n + qa];
+            vb[i] = (float)src[(size_t)i * n + qb];
+        }
+        put(va, vb, kb);
+    }
+    for (int d = 0; d < c.ob_nkb; ++d) {
+        const float* row = io.in + (size_t)(c.ob_in + d) * n;
+        float va[8], vb[8];
+        oneblob16_bins(row[qa], h, va);
+        oneblob16_bins(row[qb], h, vb);
+        put(va, vb, c.ob_kb0 + d);
+    }
+}
+
 // Queries [q_begin, q_end) of SoA arrays with row stride n.
 // One 64-query tile [q0, min(q0 + 64, q_end)) of a query stream with row stride n:
 // encode, MLP on MFMA (weights in LDS at sw), mixture, sample and pdfs.
@@ -130,9 +194,10 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
 #pragma unroll
     for (int t = 0; t < TH; ++t) acc[t][0] = acc[t][1] = mf::f32x16{};
     bool layer0_done = false;
-    if constexpr (!F8 && FAST) {
+    if constexpr (FAST) {
         if (c.ob_nkb > 0 && genc) {
-            layer0_oneblob16<O, TH>(c, sw, genc, io, n, qa, qb, lane, acc);
+            if constexpr (F8) layer0_oneblob16_fp8<T, TH>(c, sw, genc, io, n, qa, qb, lane, qscale[0], acc);
+            else layer0_oneblob16<O, TH>(c, sw, genc, io, n, qa, qb, lane, acc);
             layer0_done = true;
         }
     }
