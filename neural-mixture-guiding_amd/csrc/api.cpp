@@ -38,7 +38,7 @@ hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void*
 hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
                                     int n_cu, hipStream_t s);
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
-                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s);
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve);
 bool mfma_split_grid(const DevCfg& c);
 hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
@@ -160,6 +160,11 @@ void need(bool ok, const char* msg) {
 }
 
 }  // namespace
+
+// Unused dynamic LDS per grid-encode workgroup in the pipelined query path
+// (encode of chunk c + 1 beside the fused kernel of chunk c): caps the encode
+// at 5 workgroups per CU so that the fused kernel's workgroups find room.
+constexpr uint32_t kEncodeLdsReserve = 27u * 1024u;
 
 struct npmg_ctx {
     Config cfg;
@@ -354,7 +359,7 @@ struct npmg_ctx {
             const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
             if (chunks == 1u) {
                 timed(NPMG_TIME_GRID_ENCODE, s, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s),
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s, 0u),
                               "grid encode");
                 });
                 timed(NPMG_TIME_QUERY, s, [&] {
@@ -368,7 +373,7 @@ struct npmg_ctx {
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 timed(NPMG_TIME_GRID_ENCODE, aux, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux),
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux, kEncodeLdsReserve),
                               "grid encode");
                 });
                 hip_check(hipEventRecord(ev_chunk[k], aux), "event");

@@ -263,15 +263,19 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
 
 }  // namespace
 
+// lds_reserve: unused dynamic LDS per workgroup, which caps the encode's
+// workgroups per CU so that fused-kernel workgroups running beside it (the
+// pipelined query path) still find room; the encode loses nothing down to
+// about 5 workgroups per CU.
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
-                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s) {
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve) {
     if (q_end <= q_begin) return hipSuccess;
     const dim3 grid((q_end - q_begin + 255) / 256, c.n_levels);
     if (precision == PREC_BF16)
-        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, in, (__bf16*)genc,
-                           n, q_begin, q_end);
+        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), lds_reserve, s, c, (const __bf16*)table, in,
+                           (__bf16*)genc, n, q_begin, q_end);
     else if (precision == PREC_FP16)
-        hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, in,
+        hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), lds_reserve, s, c, (const _Float16*)table, in,
                            (_Float16*)genc, n, q_begin, q_end);
     else
         return hipErrorInvalidValue;
@@ -280,7 +284,7 @@ hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* 
 
 hipError_t launch_grid_encode(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
                               uint32_t n, hipStream_t s) {
-    return launch_grid_encode_range(c, precision, table, in, genc, n, 0u, n, s);
+    return launch_grid_encode_range(c, precision, table, in, genc, n, 0u, n, s, 0u);
 }
 
 bool mfma_split_grid(const DevCfg& c) {
