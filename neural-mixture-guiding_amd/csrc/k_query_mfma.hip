@@ -67,7 +67,11 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
 }
 
 // Level-major grid encoding of several queues in one persistent launch:
-// workgroups pull (level, 256-query tile) items in level-major order.
+// workgroups pull (level, 256-query tile) items in level-major order, a batch
+// of kEncodeBatch consecutive items per atomic (one counter serves every
+// workgroup; one atomic per item serialised on it and took most of the time).
+constexpr uint32_t kEncodeBatch = 16;
+
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, const T* __restrict__ table,
                                                             const QueueSet qs) {
@@ -77,38 +81,42 @@ __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, cons
     const uint32_t per_level = first[kMaxQueues];
     const uint32_t total = per_level * (uint32_t)c.n_levels;
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(qs.counter, 1u);
+        if (threadIdx.x == 0) item = atomicAdd(qs.counter, kEncodeBatch);
         __syncthreads();
-        const uint32_t w = item;
+        const uint32_t w0 = item;
         __syncthreads();
-        if (w >= total) break;
-        const int l = (int)(w / per_level);
-        const uint32_t t = w % per_level;
-        int qi = 0;
-        uint32_t base = 0;
+        if (w0 >= total) break;
+        const uint32_t w1 = min(total, w0 + kEncodeBatch);
+        for (uint32_t w = w0; w < w1; ++w) {
+            const int l = (int)(w / per_level);
+            const uint32_t t = w % per_level;
+            int qi = 0;
+            uint32_t base = 0;
 #pragma unroll
-        for (int i = 1; i < kMaxQueues; ++i)
-            if (t >= first[i] && first[i] < first[i + 1]) {
-                qi = i;
-                base = first[i];
+            for (int i = 1; i < kMaxQueues; ++i)
+                if (t >= first[i] && first[i] < first[i + 1]) {
+                    qi = i;
+                    base = first[i];
+                }
+            const uint32_t n = qs.stride[qi];
+            const uint32_t q = (t - base) * 256u + threadIdx.x;
+            if (q >= min(qs.counts[qi], n)) continue;
+            const float* in = qs.io[qi].in;
+            T* genc = static_cast<T*>(qs.genc[qi]);
+            const int b = c.enc_begin[c.grid_enc];
+            const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q],
+                        x2 = in[(size_t)(b + 2) * n + q];
+            if (c.F == 2) {
+                float f[2];
+                grid_level_features<T, 2>(c, table, l, x0, x1, x2, f);
+                genc[(size_t)(2 * l) * n + q] = (T)f[0];
+                genc[(size_t)(2 * l + 1) * n + q] = (T)f[1];
+            } else {
+                float f[4];
+                grid_level_features<T, 4>(c, table, l, x0, x1, x2, f);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) genc[(size_t)(4 * l + i) * n + q] = (T)f[i];
             }
-        const uint32_t n = qs.stride[qi];
-        const uint32_t q = (t - base) * 256u + threadIdx.x;
-        if (q >= min(qs.counts[qi], n)) continue;
-        const float* in = qs.io[qi].in;
-        T* genc = static_cast<T*>(qs.genc[qi]);
-        const int b = c.enc_begin[c.grid_enc];
-        const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
-        if (c.F == 2) {
-            float f[2];
-            grid_level_features<T, 2>(c, table, l, x0, x1, x2, f);
-            genc[(size_t)(2 * l) * n + q] = (T)f[0];
-            genc[(size_t)(2 * l + 1) * n + q] = (T)f[1];
-        } else {
-            float f[4];
-            grid_level_features<T, 4>(c, table, l, x0, x1, x2, f);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) genc[(size_t)(4 * l + i) * n + q] = (T)f[i];
         }
     }
 }

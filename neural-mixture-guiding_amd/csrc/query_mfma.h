@@ -341,7 +341,10 @@ __global__ void __launch_bounds__(256) k_query_mfma(const DevCfg c, const O* __r
 
 // Several queues in one persistent launch: each wave pulls 64-query tiles
 // from a device counter over the concatenated tiles of the queues (counts on
-// the device) until all are taken.
+// the device) until all are taken, kTileBatch consecutive tiles per atomic
+// (every wave shares the one counter, whose same-address atomics serialise).
+constexpr uint32_t kTileBatch = 4;
+
 template <class T, class O, int W, int K, int P, bool FAST>
 __global__ void __launch_bounds__(256) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
                                                            const T* __restrict__ table, const QueueSet qs,
@@ -357,20 +360,24 @@ __global__ void __launch_bounds__(256) k_query_mfma_queues(const DevCfg c, const
     queue_tile_prefix(qs, 64u, first);
     const uint32_t total = first[kMaxQueues];
     for (;;) {
-        uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(qs.counter, 1u);
-        w = __shfl(w, 0);  // wave-uniform work item; every wave leaves once the items run out
-        if (w >= total) break;
-        int qi = 0;
-        uint32_t base = 0;
+        uint32_t w0 = 0;
+        if (lane == 0) w0 = atomicAdd(qs.counter, kTileBatch);
+        w0 = __shfl(w0, 0);  // wave-uniform first tile; every wave leaves once the items run out
+        if (w0 >= total) break;
+        const uint32_t w1 = min(total, w0 + kTileBatch);
+        for (uint32_t w = w0; w < w1; ++w) {
+            int qi = 0;
+            uint32_t base = 0;
 #pragma unroll
-        for (int i = 1; i < kMaxQueues; ++i)
-            if (w >= first[i] && first[i] < first[i + 1]) {
-                qi = i;
-                base = first[i];
-            }
-        query_tile_mfma<T, O, W, K, P, FAST>(c, sw, table, static_cast<const T*>(qs.genc[qi]), qs.io[qi], qs.stride[qi],
-                                       (w - base) * 64u, min(qs.counts[qi], qs.stride[qi]), lane, qscale);
+            for (int i = 1; i < kMaxQueues; ++i)
+                if (w >= first[i] && first[i] < first[i + 1]) {
+                    qi = i;
+                    base = first[i];
+                }
+            query_tile_mfma<T, O, W, K, P, FAST>(c, sw, table, static_cast<const T*>(qs.genc[qi]), qs.io[qi],
+                                                 qs.stride[qi], (w - base) * 64u, min(qs.counts[qi], qs.stride[qi]),
+                                                 lane, qscale);
+        }
     }
 }
 
