@@ -221,6 +221,7 @@ struct npmg_ctx {
     double t_ms[kTags] = {};
     uint32_t t_count[kTags] = {};
     hipEvent_t ev_fork = nullptr, ev_chunk[kPipe] = {};
+    hipEvent_t ev_bwd = nullptr, ev_wgrad = nullptr;  // fork/join of the weight gradients (train_backward)
     // MFMA training: transposed weight fragments and the operand-type buffers
     void* bfrag_train = nullptr;
     TrainMfmaWS mws{};
@@ -245,6 +246,8 @@ struct npmg_ctx {
         if (capture) (void)hipStreamDestroy(capture);
         if (aux) (void)hipStreamDestroy(aux);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_bwd) (void)hipEventDestroy(ev_bwd);
+        if (ev_wgrad) (void)hipEventDestroy(ev_wgrad);
         for (hipEvent_t e : ev_chunk)
             if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
@@ -654,18 +657,32 @@ struct npmg_ctx {
                           "copy dout");
             hip_check(launch_train_backward_fp32(cfg.d, master, ws, denc, ws.dgrid, n, n_cu, s), "train backward");
         }
-        if (cfg.n_grid) {
+        // The weight gradients (from the stored layer inputs and output
+        // gradients) and the grid-gradient scatter (from dL/d(grid features))
+        // are independent and bound by different units (MFMA operand loads vs
+        // record bytes and LDS atomics): the weight gradients run on aux beside
+        // the scatter, and s waits for them before anything after the backward.
+        const bool fork = cfg.n_grid != 0;
+        hipStream_t sw = s;
+        if (fork) {
+            hip_check(hipEventRecord(ev_bwd, s), "event");
+            hip_check(hipStreamWaitEvent(aux, ev_bwd, 0), "stream wait");
+            sw = aux;
+        }
+        if (half())
+            hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, partials, partials_cap, dparams, n, of, sw),
+                      "weight gradient (mfma)");
+        else
+            hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, of, sw), "weight gradient");
+        if (fork) {
+            hip_check(hipEventRecord(ev_wgrad, aux), "event");
             float* gg = dparams + cfg.n_mlp;
             hip_check(launch_fill(gg, 0.0f, cfg.n_grid, s), "fill");
             hip_check(launch_grid_scatter(cfg.d, in, ws.dgrid, gg, n, cfg.deterministic_grid, scatter_scratch,
                                           scatter_cap, s),
                       "grid scatter");
+            hip_check(hipStreamWaitEvent(s, ev_wgrad, 0), "stream wait");
         }
-        if (half())
-            hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, partials, partials_cap, dparams, n, of, s),
-                      "weight gradient (mfma)");
-        else
-            hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, of, s), "weight gradient");
     }
 
     // One optimisation step on a batch: forward + adjoint, backward, Adam/EMA,
@@ -800,6 +817,8 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         hip_check(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking), "stream");
         hip_check(hipStreamCreateWithFlags(&ctx->capture, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&ctx->ev_bwd, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&ctx->ev_wgrad, hipEventDisableTiming), "event");
         for (hipEvent_t& e : ctx->ev_chunk) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         const Config& c = ctx->cfg;
         ctx->master = dalloc<float>(c.n_params);
