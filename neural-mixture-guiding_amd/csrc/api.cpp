@@ -464,8 +464,9 @@ struct npmg_ctx {
     void autotune(uint32_t n, hipStream_t s) {
         if (!half() || n == 0) return;
         const bool split = mfma_split_grid(cfg.d);
-        // synthetic queries: in [9][n], normal [3][n], u [3][n], eval_dirs [4][3][n]; outputs dir [3][n], pdf [n]
-        size_t bytes = sizeof(float) * (size_t)n * (9 + 3 + 3 + 12 + 3 + 1);
+        // synthetic queries: in [9][n], normal [3][n], u [3][n], eval_dirs [4][3][n]; outputs dir [3][n], pdf [n],
+        // eval pdfs [4][n] (the workload of a guided bounce: sample + 4 pdfs)
+        size_t bytes = sizeof(float) * (size_t)n * (9 + 3 + 3 + 12 + 3 + 1 + 4);
         float* buf = nullptr;
         hip_check(hipMalloc((void**)&buf, bytes), "hipMalloc");
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -476,7 +477,7 @@ struct npmg_ctx {
         const int old_b = tune_bpc;
         try {
             float *in = buf, *nrm = in + 9 * (size_t)n, *u = nrm + 3 * (size_t)n, *ed = u + 3 * (size_t)n;
-            float *od = ed + 12 * (size_t)n, *op = od + 3 * (size_t)n;
+            float *od = ed + 12 * (size_t)n, *op = od + 3 * (size_t)n, *oe = op + (size_t)n;
             hip_check(launch_synth_queries(0x7a11u, 0u, n, in, nrm, u, ed, s), "synth");
             QueryIO io{};
             io.in = in;
@@ -484,6 +485,9 @@ struct npmg_ctx {
             io.u = u;
             io.out_dir = od;
             io.out_pdf = op;
+            io.eval_dirs = ed;
+            io.n_eval = 4;
+            io.out_eval_pdf = oe;
             hip_check(hipEventCreate(&e0), "event");
             hip_check(hipEventCreate(&e1), "event");
             timing = false;
@@ -499,7 +503,7 @@ struct npmg_ctx {
                     tune_bpc = b;
                     query(io, n, true, s);  // warm-up
                     hip_check(hipEventRecord(e0, s), "event");
-                    for (int r = 0; r < 3; ++r) query(io, n, true, s);
+                    for (int r = 0; r < 5; ++r) query(io, n, true, s);
                     hip_check(hipEventRecord(e1, s), "event");
                     hip_check(hipEventSynchronize(e1), "event");
                     float ms = 0.0f;
