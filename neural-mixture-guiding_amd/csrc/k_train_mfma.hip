@@ -60,20 +60,40 @@ __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T*
         const uint32_t qra = q0 + (uint32_t)j, qrb = q0 + 32u + (uint32_t)j;
         const bool oka = qra < n, okb = qrb < n;
         const uint32_t qa = oka ? qra : n - 1u, qb = okb ? qrb : n - 1u;
+        // With the fast numerics and the "grid, then OneBlob(16)" layout the
+        // fragments come as in the query kernel: grid K-blocks from the
+        // level-major features as they are, OneBlob K-blocks from the three
+        // non-zero bins (mf::oneblob16_frag); the inputs are read per K-block.
+        const bool ob16 = c.numerics == NUM_FAST && c.ob_nkb > 0 && genc;
         float ina[16], inb[16];
 #pragma unroll
         for (int d = 0; d < 16; ++d) {
-            ina[d] = d < c.n_in ? in[(size_t)d * n + qa] : 0.0f;
-            inb[d] = d < c.n_in ? in[(size_t)d * n + qb] : 0.0f;
+            ina[d] = d < c.n_in && !ob16 ? in[(size_t)d * n + qa] : 0.0f;
+            inb[d] = d < c.n_in && !ob16 ? in[(size_t)d * n + qb] : 0.0f;
         }
         mf::f32x16 acc[TH][2];
 #pragma unroll
         for (int t = 0; t < TH; ++t) acc[t][0] = acc[t][1] = mf::f32x16{};
         for (int kb = 0; kb < KB0; ++kb) {
-            float va[8], vb[8];
-            encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
-            encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
-            const V ba = mf::pack8<T>(va), bb = mf::pack8<T>(vb);
+            V ba, bb;
+            if (ob16 && kb < c.ob_kb0) {
+                const T* src = genc + (size_t)(16 * kb + 8 * h) * n;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    ba[i] = src[(size_t)i * n + qa];
+                    bb[i] = src[(size_t)i * n + qb];
+                }
+            } else if (ob16 && kb < c.ob_kb0 + c.ob_nkb) {
+                const float* row = in + (size_t)(c.ob_in + kb - c.ob_kb0) * n;
+                ba = mf::oneblob16_frag<T>(row[qa], h);
+                bb = mf::oneblob16_frag<T>(row[qb], h);
+            } else {
+                float va[8], vb[8];
+                encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
+                encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
+                ba = mf::pack8<T>(va);
+                bb = mf::pack8<T>(vb);
+            }
             const int f0 = 8 * (2 * kb + h);
 #pragma unroll
             for (int e = 0; e < 8; ++e) {

@@ -26,48 +26,9 @@ namespace {
 using namespace dev;
 
 
-// Two 16-bit operands in one register, `lo` in bits 0..15 (the element order of
-// a fragment), each rounded to nearest even like mf::pack8.
-template <class T>
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-    const T a = (T)lo, b = (T)hi;
-    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
-}
-
-// Bins 8h .. 8h+7 of a 16-bin OneBlob encoding of x, as one B fragment (fast
-// numerics). The quartic kernel of radius one bin (tiny-cuda-nn
-// common_device.h:533-548), centred at s = 16 x = kx + t with t in [0, 1),
-// puts all its mass into bins kx - 1, kx and kx + 1, wrapped mod 16 like
-// kernel_one_blob_soa (oneblob.h:114-123): 1/2 - S(t), S(t) + S(1 - t) and
-// 1/2 - S(1 - t), with S(u) = 15/16 (u - 2u^3/3 + u^5/5) the odd part of the
-// kernel's CDF. Bins b, b + 1 are the 32-bit window at 16-bit slot
-// e = (b - kx + 2) mod 16 of the slot sequence (0, m[kx-1], m[kx], m[kx+1], 0, ...).
-template <class T>
-__device__ __forceinline__ typename mf::Frag<T>::vec oneblob16_frag(float x, int h) {
-    const float s = x * 16.0f;
-    const float fl = __builtin_floorf(s);
-    const float t = s - fl;
-    const int kx = (int)fl;
-    auto odd = [](float u) {
-        const float u2 = u * u;
-        return u * __builtin_fmaf(u2, __builtin_fmaf(u2, 0.1875f, -0.625f), 0.9375f);
-    };
-    const float st = odd(t), sr = odd(1.0f - t);
-    const uint64_t slots = ((uint64_t)pack2<T>(st + sr, 0.5f - sr) << 32) | (uint64_t)pack2<T>(0.0f, 0.5f - st);
-    const int base = 8 * h + 2 - kx;
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 d;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t e = (uint32_t)(base + 2 * j) & 15u;
-        d[j] = e < 4u ? (uint32_t)(slots >> (16u * e)) : 0u;
-    }
-    return __builtin_bit_cast(typename mf::Frag<T>::vec, d);
-}
-
 // Layer 0 of the "grid, then OneBlob(16)" layout (DevCfg::ob_nkb > 0) for a
 // 16-bit operand type: the grid K-blocks straight from the level-major
-// features, then one K-block of oneblob16_frag bins per OneBlob input dim.
+// features, then one K-block of mf::oneblob16_frag bins per OneBlob input dim.
 template <class T, int TH>
 __device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, const T* __restrict__ genc,
                                                  const QueryIO& io, uint32_t n, uint32_t qa, uint32_t qb, int lane,
@@ -91,7 +52,7 @@ __device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, c
     }
     for (int d = 0; d < c.ob_nkb; ++d) {
         const float* row = io.in + (size_t)(c.ob_in + d) * n;
-        const V ba = oneblob16_frag<T>(row[qa], h), bb = oneblob16_frag<T>(row[qb], h);
+        const V ba = mf::oneblob16_frag<T>(row[qa], h), bb = mf::oneblob16_frag<T>(row[qb], h);
         const int kb = c.ob_kb0 + d;
 #pragma unroll
         for (int t = 0; t < TH; ++t) {

@@ -169,3 +169,4 @@ def test_fast_oneblob_matches_exact_features(gpu):
     assert np.mean(rel == 0) > 0.95 and np.quantile(rel, 0.999) < 2e-2
     for n in nets:
         n.close()
+

@@ -143,6 +143,48 @@ def test_mfma_backward(gpu, oracle, prec, name):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_backward_fast_oneblob(gpu, oracle, prec):
+    """With the fast numerics the training forward builds the OneBlob fragments
+    from the three non-zero bins of the quartic kernel, as the fused query
+    kernel does. The gradients meet the same FP64 bounds as with the exact
+    bins (test_mfma_backward). Against the emulation of the exact bins'
+    rounding points they differ only where a bin's FP32 value rounds to a
+    different operand value (rare ties), so that bound is wider."""
+    import torch
+    import npmg
+    import pyoracle
+
+    cfg = npmg.benchmark_config("full", precision=prec)
+    cfg["mixture"] = dict(cfg["mixture"], numerics="fast")
+    net = npmg.GuidingNetwork(cfg, seed=7272)
+    oc = pyoracle.make_cfg(cfg)
+    n = 2000
+    q = npmg.synth_queries(8, n)
+    rng = np.random.default_rng(3)
+    dout = rng.normal(size=(net.padded_output, n)).astype(np.float32)
+    dparams, _ = net.backward(q["inputs"], torch.from_numpy(dout).cuda())
+    g = _np(dparams)
+    p = _np(net.get_params(0))
+    inp = _np(q["inputs"])
+    emu = _chain(oracle, oc, net, p, inp, prec)
+    dw_e, denc_e = _backward(emu, dout)
+    e_w, e_grid = _rel2(g[:net.n_mlp], dw_e), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_e))
+    ref_m = _chain(oracle, oc, net, p, inp, None, masks=emu["masks"])
+    dw_m, denc_m = _backward(ref_m, dout)
+    m_w, m_grid = _rel2(g[:net.n_mlp], dw_m), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_m))
+    ref = _chain(oracle, oc, net, p, inp, None)
+    dw_r, denc_r = _backward(ref, dout)
+    f_w, f_grid = _rel2(g[:net.n_mlp], dw_r), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_r))
+    print(f"{prec} fast OneBlob: emulation of the exact bins dW {e_w:.1e} grid {e_grid:.1e} | "
+          f"FP64 same masks dW {m_w:.1e} grid {m_grid:.1e} | FP64 dW {f_w:.1e} grid {f_grid:.1e}")
+    # measured: emulation 5.5e-5 (bf16) / 3.3e-5 (fp16), FP64 as with the exact bins
+    assert e_w < 1e-3 and e_grid < 1e-3
+    assert m_w < 1e-2 and m_grid < 1e-2
+    assert f_w < FLIP_BOUND[prec] and f_grid < FLIP_BOUND[prec]
+    net.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_mfma_train_step_gradient(gpu, oracle, prec):
     """The train step's loss and gradient (adjoint + backward) against FP64."""
     import npmg
