@@ -305,6 +305,8 @@ def main():
         return
     precs = [args.precision] if args.precision == "fp32" else ["fp32", args.precision]
     if args.precision != "fp32" and args.only != "query":
+        if args.precision != "fp16":
+            precs.append("fp16")
         precs.append("fp8")  # inference only
     for prec in precs:
         netp, ms = measure_queries(prec)

@@ -218,13 +218,56 @@ __device__ __forceinline__ int level_corners(const DevCfg& c, int l, float x0, f
         w[0] = 1.0f;
         return 1;
     }
+    // grid_index of the 8 corners with its decisions taken once per level
+    // (uniform): the stride walk (dims that fit the level), dense or hashed,
+    // and the hash products formed incrementally, (p + 1) m = p m + m in
+    // uint32 arithmetic, so every index is bit-identical to grid_index's.
+    uint32_t stride = 1u, s1 = 0u, s2 = 0u;
+    if (stride <= hsize) stride *= res;
+    if (stride <= hsize) {
+        s1 = stride;
+        stride *= res;
+    }
+    if (s1 != 0u && stride <= hsize) {
+        s2 = stride;
+        stride *= res;
+    }
+    const bool pow2 = (hsize & (hsize - 1u)) == 0u;
+    if (c.grid_type == GRID_HASH && hsize < stride) {
+        uint32_t m0 = 1u, m1 = 2654435761u, m2 = 805459861u;
+        if (c.hash_type == HASH_PRIME) m0 = 1958374283u;
+        else if (c.hash_type == HASH_REVERSED_PRIME) {
+            m0 = 2165219737u;
+            m1 = 1434869437u;
+            m2 = 2097192037u;
+        }
+        const uint32_t hx0 = c0 * m0, hx1 = hx0 + m0, hy0 = c1 * m1, hy1 = hy0 + m1, hz0 = c2 * m2, hz1 = hz0 + m2;
+        const uint32_t yz[4] = {hy0 ^ hz0, hy1 ^ hz0, hy0 ^ hz1, hy1 ^ hz1};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t hv = ((k & 1) ? hx1 : hx0) ^ yz[k >> 1];
+            idx[k] = pow2 ? (hv & (hsize - 1u)) : (hv % hsize);
+        }
+    } else {
+        const uint32_t base = c0 + c1 * s1 + c2 * s2;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t i = base + ((k & 1) ? 1u : 0u) + ((k & 2) ? s1 : 0u) + ((k & 4) ? s2 : 0u);
+            uint32_t r;
+            if (pow2) r = i & (hsize - 1u);
+            else if (i < hsize) r = i;
+            else if (i - hsize < hsize) r = i - hsize;
+            else r = i % hsize;
+            idx[k] = r;
+        }
+    }
+    const float a0 = 1.0f - f0, a1 = 1.0f - f1, a2 = 1.0f - f2;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         float wk = 1.0f;
-        wk *= (k & 1) ? f0 : 1.0f - f0;
-        wk *= (k & 2) ? f1 : 1.0f - f1;
-        wk *= (k & 4) ? f2 : 1.0f - f2;
-        idx[k] = grid_index(c, hsize, res, c0 + ((k & 1) ? 1u : 0u), c1 + ((k & 2) ? 1u : 0u), c2 + ((k & 4) ? 1u : 0u));
+        wk *= (k & 1) ? f0 : a0;
+        wk *= (k & 2) ? f1 : a1;
+        wk *= (k & 4) ? f2 : a2;
         w[k] = wk;
     }
     return 8;
