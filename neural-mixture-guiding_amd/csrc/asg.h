@@ -205,9 +205,18 @@ __device__ __forceinline__ S asg_eval(const AsgLobe<S>& L, V3 w) {
     const S zl = w.x * L.z[0] + w.y * L.z[1] + w.z * L.z[2];
     const S ux = F::div(xl, L.a), uy = yl * L.a;
     const S s2 = ux * ux + uy * uy + zl * zl;
-    S cm1 = F::div(zl, F::sqrt(s2)) - s_lift(zl, 1.0f);  // cos - 1 in the unstretched frame, at most 0
-    if (s_val(cm1) > 0.0f) cm1 = s_lift(cm1, 0.0f);
-    return F::exp(L.kappa * cm1 + L.lognorm - 1.5f * F::log(s2));
+    if constexpr (__is_same(S, float) && __is_same(MT, FastMath)) {
+        // one reciprocal square root r = |u|^-1 gives the cosine and the
+        // Jacobian |u|^-3 = r^3 (s2 lies in [1/64, 64] for a in [1/8, 8])
+        const float r = __builtin_amdgcn_rsqf(s2);
+        float cm1 = zl * r - 1.0f;
+        if (cm1 > 0.0f) cm1 = 0.0f;
+        return MT::exp(L.kappa * cm1 + L.lognorm) * (r * r * r);
+    } else {
+        S cm1 = F::div(zl, F::sqrt(s2)) - s_lift(zl, 1.0f);  // cos - 1 in the unstretched frame, at most 0
+        if (s_val(cm1) > 0.0f) cm1 = s_lift(cm1, 0.0f);
+        return F::exp(L.kappa * cm1 + L.lognorm - 1.5f * F::log(s2));
+    }
 }
 
 // The shape outputs of lobe i as S, from a raw-output accessor.
