@@ -208,17 +208,30 @@ hipError_t launch_debug_math(int fn, const float* x, const float* y, uint32_t n,
 }
 
 namespace {
+// Grid-stride fill with 16-byte stores: the `head` floats before the first
+// 16-byte boundary of p and the `tail` floats after the last whole float4 are
+// stored singly by the first threads.
 __global__ void __launch_bounds__(256) k_fill(float* p, float v, size_t n) {
-    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
-    if (i < n) p[i] = v;
+    const size_t tid = (size_t)blockIdx.x * 256u + threadIdx.x, stride = (size_t)gridDim.x * 256u;
+    const size_t to_boundary = (4u - (((uintptr_t)p >> 2) & 3u)) & 3u;
+    const size_t head = to_boundary < n ? to_boundary : n;
+    const size_t n4 = (n - head) / 4u, tail = n - head - 4u * n4;
+    if (tid < head) p[tid] = v;
+    if (tid < tail) p[head + 4u * n4 + tid] = v;
+    float4* q = reinterpret_cast<float4*>(p + head);
+    const float4 v4 = make_float4(v, v, v, v);
+    for (size_t i = tid; i < n4; i += stride) q[i] = v4;
 }
 }  // namespace
 
 // Fill kernel used instead of hipMemsetAsync so that the train step stays
-// safe to capture in a HIP graph.
+// safe to capture in a HIP graph. At most 4096 workgroups (16 per CU), each
+// thread storing every stride-th float4.
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    const size_t blocks = (n / 4u + 255u) / 256u;
+    const unsigned grid = (unsigned)(blocks < 1u ? 1u : (blocks > 4096u ? 4096u : blocks));
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 
