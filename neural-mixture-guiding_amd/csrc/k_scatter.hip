@@ -5,7 +5,7 @@
 // Every (sample, level, corner) contributes w_k * dL/dy_{l,f} to one table
 // entry. The contributions are binned by (level, tile of 8192 / F entries),
 // so that one tile of gradients fits in LDS:
-//   1. count    per (bin, block of 4096 samples), histogram in LDS;
+//   1. count    per (bin, block of 2048 samples), histogram in LDS;
 //   2. scan     per bin over its blocks, then over bins: each (bin, block)
 //               owns a contiguous run of records, in block order; the bins
 //               are cut into chunks of at most kChunk records;
@@ -37,7 +37,7 @@ namespace {
 
 using namespace dev;
 
-constexpr uint32_t kSamplesPerBlock = 4096;
+constexpr uint32_t kSamplesPerBlock = 2048;
 constexpr uint32_t kTileWords = 8192;  // words of one tile: tile entries x F
 constexpr uint32_t kChunk = 65536;     // records reduced by one workgroup
 constexpr uint32_t kMaxBinsPerLevel = 8192;
