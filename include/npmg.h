@@ -73,11 +73,19 @@ npmg_status npmg_grid_layout(const npmg_ctx* ctx, uint32_t* n_levels, uint32_t* 
                              uint32_t* resolutions);
 
 /* Full-precision parameters in the tcnn order. set_params writes the training
- * master copy and the inference (EMA) copy; which = 0 reads the training
+ * master copy and the inference (EMA) copy (and resets the FP8 calibration);
+ * which = 0 reads the training
  * master, which = 1 the inference copy. Trainer::set_params_full_precision /
  * params_inference (trainer.h:314-328). */
 npmg_status npmg_set_params(npmg_ctx* ctx, const float* params, size_t n, int on_device);
 npmg_status npmg_get_params(const npmg_ctx* ctx, float* params, size_t n, int on_device, int which);
+/* Writes one copy only: which = 0 the training master (the parameters that
+ * tiny-cuda-nn's PyTorch binding passes to each forward and backward call,
+ * bindings/torch/tinycudann/modules.py:82-151, which leave the EMA alone),
+ * 1 the inference copy (the EMA; the master when there is no EMA), 2 both
+ * (npmg_set_params). Synchronous. A new inference copy resets the FP8
+ * calibration (the next inference call recalibrates). */
+npmg_status npmg_set_params_which(npmg_ctx* ctx, const float* params, size_t n, int on_device, int which);
 
 /* Network inference with the inference (EMA) weights: in [9][n] -> raw
  * network outputs [n_out][n] (Network::inference, object.h:147-169). */

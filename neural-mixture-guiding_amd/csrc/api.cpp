@@ -929,6 +929,24 @@ npmg_status npmg_set_params(npmg_ctx* ctx, const float* params, size_t n, int on
         const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         hip_check(hipMemcpy(ctx->master, params, n * sizeof(float), k), "copy params");
         if (ctx->ema) hip_check(hipMemcpy(ctx->ema, params, n * sizeof(float), k), "copy params");
+        ctx->fp8_calibrated = false;  // new inference weights: the next query recalibrates
+        ctx->refresh_layouts(nullptr);
+        hip_check(hipDeviceSynchronize(), "synchronize");
+    });
+}
+
+npmg_status npmg_set_params_which(npmg_ctx* ctx, const float* params, size_t n, int on_device, int which) {
+    if (!ctx) return NPMG_EINVAL;
+    if (which == 2) return npmg_set_params(ctx, params, n, on_device);
+    return guarded(ctx, [&] {
+        need(params && n == ctx->cfg.n_params, "params must hold exactly n_params values");
+        need(which == 0 || which == 1, "which must be 0 (training), 1 (inference) or 2 (both)");
+        const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        hip_check(hipDeviceSynchronize(), "synchronize");
+        // without an EMA the inference copy is the training master
+        float* dst = which == 1 && ctx->ema ? ctx->ema : ctx->master;
+        hip_check(hipMemcpy(dst, params, n * sizeof(float), k), "copy params");
+        if (ctx->cfg.infer_fp8 && dst == ctx->infer_params()) ctx->fp8_calibrated = false;
         ctx->refresh_layouts(nullptr);
         hip_check(hipDeviceSynchronize(), "synchronize");
     });
@@ -1228,6 +1246,7 @@ npmg_status npmg_snapshot_load(npmg_ctx* ctx, const void* buf, size_t size) {
         }
         st.overflow = 0u;
         hip_check(hipMemcpy(ctx->sstate, &st, sizeof(st), hipMemcpyHostToDevice), "upload");
+        ctx->fp8_calibrated = false;  // new inference weights: the next query recalibrates
         ctx->refresh_layouts(nullptr);
         hip_check(hipDeviceSynchronize(), "synchronize");
     });
@@ -1311,6 +1330,7 @@ npmg_status npmg_fp8_scales(npmg_ctx* ctx, float* weight_scales, float* input_sc
 }
 
 npmg_status npmg_set_tuning(npmg_ctx* ctx, uint32_t chunks, uint32_t blocks_per_cu) {
+    if (!ctx) return NPMG_EINVAL;
     return guarded(ctx, [&] {
         need(chunks >= 1u && chunks <= (uint32_t)npmg_ctx::kPipe, "pipeline chunks must be 1..8");
         need(blocks_per_cu <= 64u, "workgroups per CU must be 0..64");

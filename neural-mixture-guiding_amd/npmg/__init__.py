@@ -61,6 +61,7 @@ SIGNATURES = {
     "npmg_grid_layout": (_int, [_vp, ctypes.POINTER(_u32), _vp, _vp, _vp]),
     "npmg_set_params": (_int, [_vp, _vp, _sz, _int]),
     "npmg_get_params": (_int, [_vp, _vp, _sz, _int, _int]),
+    "npmg_set_params_which": (_int, [_vp, _vp, _sz, _int, _int]),
     "npmg_infer": (_int, [_vp, _vp, _vp, _u32, _vp]),
     "npmg_sample_pdf": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "npmg_mixture_sample_pdf": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
@@ -252,11 +253,15 @@ class GuidingNetwork:
         self._check(self._lib.npmg_get_params(self._h, _ptr(out), self.n_params, 1, which))
         return out
 
-    def set_params(self, params):
+    def set_params(self, params, which: int = 2):
+        """which: 0 the training master only, 1 the inference (EMA) copy only, 2 both."""
         import torch
 
         p = params.to(device="cuda", dtype=torch.float32).contiguous()
-        self._check(self._lib.npmg_set_params(self._h, _ptr(p), self.n_params, 1))
+        if which == 2:
+            self._check(self._lib.npmg_set_params(self._h, _ptr(p), self.n_params, 1))
+        else:
+            self._check(self._lib.npmg_set_params_which(self._h, _ptr(p), self.n_params, 1, which))
 
     # -- inference ------------------------------------------------------------
     def inference(self, inputs):

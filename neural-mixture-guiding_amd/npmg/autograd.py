@@ -3,10 +3,13 @@
 Follows the pattern of tiny-cuda-nn's PyTorch binding
 (src/ext/tcnn/bindings/torch/tinycudann/modules.py:82-151): the parameters are
 a flat float32 tensor in the library's order (MLP matrices, then the grid
-table); forward copies them into the context and runs the HIP forward;
-backward runs the hand-written HIP backward and returns dL/dparams. Inputs
-(query records) receive no gradient, as in the reference where the network
-input is data.
+table); forward copies them into the context's training master (the EMA /
+inference copy is left alone) and runs the HIP forward; backward copies the
+parameters saved at forward time back into the master (another forward, a
+set_params or an optimiser step may have changed it in between, as
+modules.py:82-151 passes the saved params to bwd) and runs the hand-written
+HIP backward, returning dL/dparams. Inputs (query records) receive no
+gradient, as in the reference where the network input is data.
 """
 from __future__ import annotations
 
@@ -16,15 +19,17 @@ import torch
 class NetworkFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, net, inputs, params):
-        net.set_params(params.detach())
+        p = params.detach()
+        net.set_params(p, which=0)
         out = net.forward(inputs.detach().contiguous(), use_inference_params=False)
         ctx.net = net
-        ctx.save_for_backward(inputs)
+        ctx.save_for_backward(inputs, p)
         return out
 
     @staticmethod
     def backward(ctx, dout):
-        (inputs,) = ctx.saved_tensors
+        inputs, params = ctx.saved_tensors
+        ctx.net.set_params(params, which=0)
         dparams, _ = ctx.net.backward(inputs.contiguous(), dout.contiguous().float())
         return None, None, dparams
 

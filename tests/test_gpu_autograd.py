@@ -27,10 +27,10 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _cfg(prec):
+def _cfg(prec, size="small"):
     import npmg
 
-    c = npmg.benchmark_config("small", precision=prec)
+    c = npmg.benchmark_config(size, precision=prec)
     c["loss"] = {"scale": 128.0, "grad_clamp": 0.0}
     c["grid_gradient"] = "deterministic"  # fixed summation order: repeatable bits
     return c
@@ -57,13 +57,16 @@ def _central(oracle, oc, n_mlp, p, h, g, direction):
     return abs(g @ disp - dl) / abs(dl)
 
 
-def test_autograd_gradient_matches_fp64_differences(gpu, oracle):
+@pytest.mark.parametrize("size", ["small", "full"])
+def test_autograd_gradient_matches_fp64_differences(gpu, oracle, size):
+    """size "full": the network of the full-frame configuration (16 levels of
+    2^19 entries, MLP 3 x 64), the differentiation check's own network."""
     import torch
     import npmg
     import pyoracle
     from npmg.autograd import mixture_loss, network_apply
 
-    cfg = _cfg("fp32")
+    cfg = _cfg("fp32", size)
     net = npmg.GuidingNetwork(cfg, seed=31)
     oc = pyoracle.make_cfg(cfg)
     t = npmg.synth_train(13, N)
@@ -98,7 +101,8 @@ def test_autograd_gradient_matches_fp64_differences(gpu, oracle):
         v = rng.normal(size=net.n_params)
         v /= np.linalg.norm(v)
         dworst = max(dworst, _central(oracle, oc, net.n_mlp, p, h, g, v))
-    print(f"loss {loss.item():.6e} (FP64 {l64:.6e}); coordinate worst {worst:.2e}, direction worst {dworst:.2e}")
+    print(f"{size}: loss {loss.item():.6e} (FP64 {l64:.6e}); coordinate worst {worst:.2e}, "
+          f"direction worst {dworst:.2e}")
     assert worst < 1e-3 and dworst < 1e-3
     net.close()
 
@@ -124,4 +128,42 @@ def test_autograd_equals_split_passes(gpu, prec):
     (2.0 * mixture_loss(net, network_apply(net, t["inputs"], params2), t["tgt_dir"], t["tgt_rgb"],
                         t["tgt_pdf"])).backward()
     assert np.array_equal(_np(params2.grad), 2.0 * _np(params.grad))
+    net.close()
+
+
+def test_autograd_backward_uses_forward_time_params(gpu):
+    """The backward differentiates at the parameters its forward saw, even when
+    another forward with other parameters (or an optimiser step) ran in
+    between, and network_apply leaves the inference (EMA) copy alone."""
+    import torch
+    import npmg
+    from npmg.autograd import mixture_loss, network_apply
+
+    cfg = _cfg("fp32")
+    cfg["optimizer"] = {"otype": "Ema", "decay": 0.99, "nested": {"otype": "Adam", "learning_rate": 1e-2,
+                                                                  "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
+                                                                  "l2_reg": 1e-6}}
+    net = npmg.GuidingNetwork(cfg, seed=21)
+    t = npmg.synth_train(6, 2048)
+    p_a = net.get_params(0).clone()
+    ema0 = net.get_params(1).clone()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    p_b = p_a + 1e-2 * torch.randn(p_a.shape, device="cuda", generator=gen)
+    pa = p_a.clone().requires_grad_(True)
+    pb = p_b.clone().requires_grad_(True)
+    loss_a = mixture_loss(net, network_apply(net, t["inputs"], pa), t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])
+    loss_b = mixture_loss(net, network_apply(net, t["inputs"], pb), t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])
+    assert torch.equal(net.get_params(1), ema0), "network_apply changed the inference copy"
+    net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"])  # moves the master again
+    loss_a.backward()
+    loss_b.backward()
+    # split passes at each parameter vector
+    for p, got, l in ((p_a, pa.grad, loss_a), (p_b, pb.grad, loss_b)):
+        net.set_params(p, which=0)
+        raw = net.forward(t["inputs"])
+        dout, per = net.mixture_adjoint(raw, t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], 1.0)
+        want, _ = net.backward(t["inputs"], dout)
+        assert l.item() == per.mean().item()
+        assert np.array_equal(_np(got).view(np.uint32), _np(want).view(np.uint32))
+    assert not torch.equal(pa.grad, pb.grad)
     net.close()
