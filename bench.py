@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16", "bf16x3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="all", choices=["all", "query", "frame"],
                     help="query: time only the fused query kernel (for counter runs); "
@@ -305,8 +305,9 @@ def main():
         return
     precs = [args.precision] if args.precision == "fp32" else ["fp32", args.precision]
     if args.precision != "fp32" and args.only != "query":
-        if args.precision != "fp16":
-            precs.append("fp16")
+        for extra in ("bf16", "fp16", "bf16x3"):
+            if extra != args.precision:
+                precs.append(extra)
         precs.append("fp8")  # inference only
     for prec in precs:
         netp, ms = measure_queries(prec)
@@ -407,11 +408,11 @@ def main():
         "queries_per_s_stable_numerics": {k: N_QUERIES / (v * 1e-3) for k, v in ms_exact.items()},
         "train_samples_per_s_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_train.items()},
         "train_samples_per_s_graph_by_precision": {k: N_TRAIN / (v * 1e-3) for k, v in per_prec_graph.items()},
-        "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16")},
+        "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16", "bf16x3")},
     }
     if half:
         result["pdf_rel_diff_vs_fp32_same_params"] = {
-            p: pdf_agreement(full_cfg, p, True) for p in ("bf16", "fp16", "fp8")}
+            p: pdf_agreement(full_cfg, p, True) for p in ("bf16", "fp16", "bf16x3", "fp8")}
         result["fp8_inference_speedup_vs_fp32"] = per_prec["fp32"] / per_prec["fp8"]
         frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
         frame["workload"] = ("one frame of 8 bounce queues of floor(2,073,600 * 0.68^b) queries, one persistent "

@@ -260,7 +260,11 @@ struct npmg_ctx {
             if (p) (void)hipFree(p);
     }
 
-    bool half() const { return cfg.precision == PREC_BF16 || cfg.precision == PREC_FP16; }
+    bool half() const {
+        return cfg.precision == PREC_BF16 || cfg.precision == PREC_FP16 || cfg.precision == PREC_BF16X3;
+    }
+    // planes of an MFMA operand buffer: 2 (hi, lo) for the split BF16 operands
+    size_t planes() const { return cfg.precision == PREC_BF16X3 ? 2u : 1u; }
 
     // Brackets the launches enqueued by f on stream s with a timing event pair (tag).
     template <class F>
@@ -590,10 +594,10 @@ struct npmg_ctx {
         if (d.grid_enc >= 0) ws.dgrid = dalloc<float>((size_t)d.n_levels * d.F * n);
         size_t need;
         if (half()) {
-            mws.x0 = dalloc<uint16_t>((size_t)d.enc_padded * n);
-            mws.xh = dalloc<uint16_t>(hw);
-            mws.dyh = dalloc<uint16_t>(hw);
-            mws.dyo = dalloc<uint16_t>((size_t)d.n_out_pad * n);
+            mws.x0 = dalloc<uint16_t>((size_t)d.enc_padded * n * planes());
+            mws.xh = dalloc<uint16_t>(hw * planes());
+            mws.dyh = dalloc<uint16_t>(hw * planes());
+            mws.dyo = dalloc<uint16_t>((size_t)d.n_out_pad * n * planes());
             mws.mask = dalloc<uint32_t>(train_mask_words(d, n));
             mws.raw = dalloc<float>((size_t)d.n_out_pad * n);
             mws.dgrid = ws.dgrid;
@@ -835,12 +839,15 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         ctx->wt_infer = dalloc<float>(c.n_mlp);
         if (ctx->half()) {
             need(c.n_mlp % 512 == 0, "half precision needs layer sizes that are multiples of 32 x 16");
-            ctx->frag_train = dalloc<uint16_t>(c.n_mlp);
+            // the split operands keep two planes of weight fragments in LDS
+            need(c.precision != PREC_BF16X3 || c.d.width == 64, "precision bf16x3 needs a network of width 64");
+            ctx->frag_train = dalloc<uint16_t>(c.n_mlp * ctx->planes());
             if (c.infer_fp8) ctx->frag_infer = dalloc<uint8_t>(c.n_mlp);
-            else ctx->frag_infer = dalloc<uint16_t>(c.n_mlp);
+            else ctx->frag_infer = dalloc<uint16_t>(c.n_mlp * ctx->planes());
             ctx->htab_train = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
             ctx->htab_infer = dalloc<uint16_t>(std::max<size_t>(c.n_grid, 1));
-            if (mfma_train_supported(c.d)) ctx->bfrag_train = dalloc<uint16_t>(bfrag_elements(c.d));
+            if (mfma_train_supported(c.d))
+                ctx->bfrag_train = dalloc<uint16_t>(bfrag_elements(c.d) * ctx->planes());
         }
         ctx->sstate = dalloc<ScaleState>(1);
         ctx->mean_partials = dalloc<float>(256);

@@ -245,8 +245,9 @@ Config parse_config(const std::string& text) {
     if (ieq(prec, "fp32")) c.precision = PREC_FP32;
     else if (ieq(prec, "bf16")) c.precision = PREC_BF16;
     else if (ieq(prec, "fp16")) c.precision = PREC_FP16;
+    else if (ieq(prec, "bf16x3")) c.precision = PREC_BF16X3;
     else if (ieq(prec, "fp8")) bad("fp8 is inference-only: precision bf16 with inference_precision fp8");
-    else bad("precision must be fp32, bf16 or fp16");
+    else bad("precision must be fp32, bf16, fp16 or bf16x3");
     const std::string iprec = root.string("inference_precision", "same");
     if (ieq(iprec, "fp8")) {
         if (c.precision != PREC_BF16) bad("inference_precision fp8 needs precision bf16");

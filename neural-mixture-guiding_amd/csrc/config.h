@@ -17,7 +17,9 @@ enum Mapping : int { MAP_REFERENCE = 0, MAP_SOFTMAX = 1 };
 // (the training adjoint keeps the exact functions).
 enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1, NUM_FAST = 2 };
 enum MixtureType : int { MIX_VMF = 0, MIX_ASG = 1 };
-enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3 };
+// PREC_BF16X3: MFMA on split BF16 operands (hi + lo, three products) with the
+// grid table and level-major features in FP16.
+enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3, PREC_BF16X3 = 4 };
 
 constexpr int kMaxEnc = 4;
 constexpr int kMaxLevels = 32;

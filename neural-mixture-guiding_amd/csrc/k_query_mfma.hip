@@ -124,8 +124,11 @@ __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, cons
 // Weight fragments from full-precision row-major matrices: fragment
 // (layer, t, kb), lane (h, i), element e holds W_l[32 t + i][col] with
 // col = 16 kb + 8 h + e on the first layer and the chained permutation after.
+// Split-BF16 operands (mf::Bf16x3): fragment f is stored as the hi fragment
+// (2 f) and the lo fragment (2 f + 1) of BF16 elements.
 template <class T>
-__global__ void __launch_bounds__(256) k_pack_frags(const DevCfg c, const float* __restrict__ w, T* __restrict__ frag) {
+__global__ void __launch_bounds__(256) k_pack_frags(const DevCfg c, const float* __restrict__ w,
+                                                    typename mf::Op<T>::elem* __restrict__ frag) {
     const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     uint32_t base = 0;
     for (int l = 0; l < c.n_layers; ++l) {
@@ -139,7 +142,15 @@ __global__ void __launch_bounds__(256) k_pack_frags(const DevCfg c, const float*
             const uint32_t t = f / kbs, kb = f % kbs;
             const uint32_t h = lane / 32u, i = lane % 32u;
             const uint32_t col = l == 0 ? 16u * kb + 8u * h + e : (uint32_t)mf::chained_column((int)kb, (int)h, (int)e);
-            frag[gid] = (T)w[c.layer_off[l] + (32u * t + i) * iw + col];
+            const float x = w[c.layer_off[l] + (32u * t + i) * iw + col];
+            if constexpr (mf::Op<T>::planes == 2) {
+                const size_t g0 = (size_t)(base / 512u + f) * 1024u + within;
+                const __bf16 hi = (__bf16)x;
+                frag[g0] = hi;
+                frag[g0 + 512u] = (__bf16)(x - (float)hi);
+            } else {
+                frag[gid] = (T)x;
+            }
             return;
         }
         base += cnt;
@@ -237,11 +248,13 @@ hipError_t launch_mfma_queues_type(const DevCfg& c, const void* wfrag, const voi
     NPMG_QQ(64, 7, 4)
     NPMG_QQ(64, 8, 4)
     NPMG_QQ(64, 16, 4)
-    NPMG_QQ(128, 7, 4)
-    NPMG_QQ(128, 8, 4)
-    NPMG_QQ(128, 16, 4)
     NPMG_QQ(64, 16, 6)
-    NPMG_QQ(128, 16, 6)
+    if constexpr (mf::Op<O>::planes == 1) {  // split weights of a 128-wide network do not fit in LDS
+        NPMG_QQ(128, 7, 4)
+        NPMG_QQ(128, 8, 4)
+        NPMG_QQ(128, 16, 4)
+        NPMG_QQ(128, 16, 6)
+    }
 #undef NPMG_QQ
     return hipErrorInvalidValue;
 }
@@ -260,11 +273,13 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
     NPMG_QM(64, 7, 4)
     NPMG_QM(64, 8, 4)
     NPMG_QM(64, 16, 4)
-    NPMG_QM(128, 7, 4)
-    NPMG_QM(128, 8, 4)
-    NPMG_QM(128, 16, 4)
     NPMG_QM(64, 16, 6)
-    NPMG_QM(128, 16, 6)
+    if constexpr (mf::Op<O>::planes == 1) {  // split weights of a 128-wide network do not fit in LDS
+        NPMG_QM(128, 7, 4)
+        NPMG_QM(128, 8, 4)
+        NPMG_QM(128, 16, 4)
+        NPMG_QM(128, 16, 6)
+    }
 #undef NPMG_QM
     return hipErrorInvalidValue;
 }
@@ -282,7 +297,7 @@ hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* 
     if (precision == PREC_BF16)
         hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), lds_reserve, s, c, (const __bf16*)table, in,
                            (__bf16*)genc, n, q_begin, q_end);
-    else if (precision == PREC_FP16)
+    else if (precision == PREC_FP16 || precision == PREC_BF16X3)
         hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), lds_reserve, s, c, (const _Float16*)table, in,
                            (_Float16*)genc, n, q_begin, q_end);
     else
@@ -318,6 +333,9 @@ hipError_t launch_query_mfma(const DevCfg& c, int precision, const void* wfrag, 
     if (precision == PREC_FP16)
         return launch_mfma_type<_Float16, _Float16>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu,
                                                     blocks_per_cu, nullptr, s);
+    if (precision == PREC_BF16X3)
+        return launch_mfma_type<_Float16, mf::Bf16x3>(c, wfrag, table, genc, io, n, q_begin, q_end, n_cu,
+                                                      blocks_per_cu, nullptr, s);
     return hipErrorInvalidValue;
 }
 
@@ -335,6 +353,8 @@ hipError_t launch_query_mfma_queues(const DevCfg& c, int precision, const void* 
         return launch_mfma_queues_type<__bf16, __bf16>(c, wfrag, table, qs, n_cu, blocks_per_cu, nullptr, s);
     if (precision == PREC_FP16)
         return launch_mfma_queues_type<_Float16, _Float16>(c, wfrag, table, qs, n_cu, blocks_per_cu, nullptr, s);
+    if (precision == PREC_BF16X3)
+        return launch_mfma_queues_type<_Float16, mf::Bf16x3>(c, wfrag, table, qs, n_cu, blocks_per_cu, nullptr, s);
     return hipErrorInvalidValue;
 }
 
@@ -345,7 +365,7 @@ hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void*
     const dim3 grid((uint32_t)n_cu * 8u);
     if (precision == PREC_BF16)
         hipLaunchKernelGGL((k_grid_encode_queues<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, qs);
-    else if (precision == PREC_FP16)
+    else if (precision == PREC_FP16 || precision == PREC_BF16X3)
         hipLaunchKernelGGL((k_grid_encode_queues<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, qs);
     else
         return hipErrorInvalidValue;
@@ -383,6 +403,8 @@ hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, voi
     const dim3 grid((n + 255) / 256);
     if (precision == PREC_BF16) hipLaunchKernelGGL((k_pack_frags<__bf16>), grid, dim3(256), 0, s, c, w, (__bf16*)frag);
     else if (precision == PREC_FP16) hipLaunchKernelGGL((k_pack_frags<_Float16>), grid, dim3(256), 0, s, c, w, (_Float16*)frag);
+    else if (precision == PREC_BF16X3)
+        hipLaunchKernelGGL((k_pack_frags<mf::Bf16x3>), grid, dim3(256), 0, s, c, w, (__bf16*)frag);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -391,7 +413,8 @@ hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, 
     if (n == 0) return hipSuccess;
     const dim3 grid((unsigned)((n + 255) / 256));
     if (precision == PREC_BF16) hipLaunchKernelGGL((k_to_half<__bf16>), grid, dim3(256), 0, s, src, (__bf16*)dst, n);
-    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_to_half<_Float16>), grid, dim3(256), 0, s, src, (_Float16*)dst, n);
+    else if (precision == PREC_FP16 || precision == PREC_BF16X3)
+        hipLaunchKernelGGL((k_to_half<_Float16>), grid, dim3(256), 0, s, src, (_Float16*)dst, n);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -15,6 +15,8 @@
 // Stored for the weight gradients (operand type, [dim][n]): layer inputs and
 // output gradients of every layer; dL/d(grid features) [L F][n] in FP32 for the
 // grid scatter.
+#include <type_traits>
+
 #include "device.h"
 #include "kernels.h"
 #include "mfma.h"
@@ -25,32 +27,42 @@ namespace {
 
 using namespace dev;
 
+// (a split value is finite when its hi part is: lo = x - hi)
 template <class T>
 __device__ __forceinline__ bool finite_v(const typename mf::Frag<T>::vec& v) {
     bool ok = true;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) ok = ok && __builtin_isfinite((float)v[e]);
+    for (int e = 0; e < 8; ++e) ok = ok && __builtin_isfinite((float)mf::get<T>(v, 0, e));
     return ok;
 }
 
 // Forward chain; mask word (tile, l, t, lane) bit 16 cc + r = ReLU active for
 // accumulator register r of half cc of column tile t.
+// T is the operand type; G = mf::Op<T>::table the type of the grid table and
+// features; the stored operands are mf::Op<T>::planes planes of elements E
+// (hi then lo for the split type) at a stride of one whole buffer.
 template <class T, int W, int NOUT, int H>
 __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T* __restrict__ wfrag,
-                                                        const T* __restrict__ table, const T* __restrict__ genc,
+                                                        const typename mf::Op<T>::table* __restrict__ table,
+                                                        const typename mf::Op<T>::table* __restrict__ genc,
                                                         const float* __restrict__ in, const TrainMfmaWS ws, uint32_t n,
                                                         uint32_t n_frag) {
     using V = typename mf::Frag<T>::vec;
+    using E = typename mf::Op<T>::elem;
+    using G = typename mf::Op<T>::table;
+    constexpr int NP = mf::Op<T>::planes;
     constexpr int TO = NOUT / 32;
     constexpr int TH = W / 32;
     constexpr int KBH = W / 16;
     static_assert(H >= 1 && NOUT % 32 == 0, "shape");
     extern __shared__ uint4 smem_frag[];
-    for (uint32_t i = threadIdx.x; i < n_frag * 64u; i += 256u) smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
+    for (uint32_t i = threadIdx.x; i < n_frag * 64u * NP; i += 256u)
+        smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
     __syncthreads();
     const T* sw = reinterpret_cast<const T*>(smem_frag);
-    T* x0 = static_cast<T*>(ws.x0);
-    T* xh = static_cast<T*>(ws.xh);
+    E* x0 = static_cast<E*>(ws.x0);
+    E* xh = static_cast<E*>(ws.xh);
+    const size_t px0 = (size_t)c.enc_padded * n, pxh = (size_t)H * W * n;  // plane strides
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     const int KB0 = c.enc_padded / 16;
@@ -77,11 +89,22 @@ __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T*
         for (int kb = 0; kb < KB0; ++kb) {
             V ba, bb;
             if (ob16 && kb < c.ob_kb0) {
-                const T* src = genc + (size_t)(16 * kb + 8 * h) * n;
+                const G* src = genc + (size_t)(16 * kb + 8 * h) * n;
+                if constexpr (std::is_same_v<G, T>) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    ba[i] = src[(size_t)i * n + qa];
-                    bb[i] = src[(size_t)i * n + qb];
+                    for (int i = 0; i < 8; ++i) {
+                        ba[i] = src[(size_t)i * n + qa];
+                        bb[i] = src[(size_t)i * n + qb];
+                    }
+                } else {
+                    float va[8], vb[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        va[i] = (float)src[(size_t)i * n + qa];
+                        vb[i] = (float)src[(size_t)i * n + qb];
+                    }
+                    ba = mf::pack8<T>(va);
+                    bb = mf::pack8<T>(vb);
                 }
             } else if (ob16 && kb < c.ob_kb0 + c.ob_nkb) {
                 const float* row = in + (size_t)(c.ob_in + kb - c.ob_kb0) * n;
@@ -89,17 +112,19 @@ __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T*
                 bb = mf::oneblob16_frag<T>(row[qb], h);
             } else {
                 float va[8], vb[8];
-                encode_chunk<T>(c, table, ina, 2 * kb + h, va, genc, qa, n);
-                encode_chunk<T>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
+                encode_chunk<G>(c, table, ina, 2 * kb + h, va, genc, qa, n);
+                encode_chunk<G>(c, table, inb, 2 * kb + h, vb, genc, qb, n);
                 ba = mf::pack8<T>(va);
                 bb = mf::pack8<T>(vb);
             }
             const int f0 = 8 * (2 * kb + h);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (oka) x0[(size_t)(f0 + e) * n + qra] = ba[e];
-                if (okb) x0[(size_t)(f0 + e) * n + qrb] = bb[e];
-            }
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    if (oka) x0[p * px0 + (size_t)(f0 + e) * n + qra] = mf::get<T>(ba, p, e);
+                    if (okb) x0[p * px0 + (size_t)(f0 + e) * n + qrb] = mf::get<T>(bb, p, e);
+                }
 #pragma unroll
             for (int t = 0; t < TH; ++t) {
                 const V a = mf::load_frag<T>(sw, t * KB0 + kb, lane);
@@ -130,9 +155,12 @@ __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T*
                     bx[cc][kb] = mf::pack8<T>(v);
                     const uint32_t qr = cc ? qrb : qra;
                     if (cc ? okb : oka) {
-                        T* dst = xh + ((size_t)(l - 1) * W) * n + qr;
+                        E* dst = xh + ((size_t)(l - 1) * W) * n + qr;
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) dst[(size_t)mf::chained_column(kb, h, e) * n] = bx[cc][kb][e];
+                        for (int p = 0; p < NP; ++p)
+#pragma unroll
+                            for (int e = 0; e < 8; ++e)
+                                dst[p * pxh + (size_t)mf::chained_column(kb, h, e) * n] = mf::get<T>(bx[cc][kb], p, e);
                     }
                 }
 #pragma unroll
@@ -181,15 +209,19 @@ __global__ void __launch_bounds__(256) k_train_bwd_mfma(const DevCfg c, const T*
                                                         float* __restrict__ denc, uint32_t n, uint32_t n_bfrag,
                                                         uint32_t* __restrict__ overflow) {
     using V = typename mf::Frag<T>::vec;
+    using E = typename mf::Op<T>::elem;
+    constexpr int NP = mf::Op<T>::planes;
     constexpr int TH = W / 32;
     constexpr int KBH = W / 16;
     constexpr int KBO = NOUT / 16;
     extern __shared__ uint4 smem_frag[];
-    for (uint32_t i = threadIdx.x; i < n_bfrag * 64u; i += 256u) smem_frag[i] = reinterpret_cast<const uint4*>(bfrag)[i];
+    for (uint32_t i = threadIdx.x; i < n_bfrag * 64u * NP; i += 256u)
+        smem_frag[i] = reinterpret_cast<const uint4*>(bfrag)[i];
     __syncthreads();
     const T* sb = reinterpret_cast<const T*>(smem_frag);
-    T* dyh = static_cast<T*>(ws.dyh);
-    T* dyo = static_cast<T*>(ws.dyo);
+    E* dyh = static_cast<E*>(ws.dyh);
+    E* dyo = static_cast<E*>(ws.dyo);
+    const size_t pdh = (size_t)H * W * n, pdo = (size_t)NOUT * n;  // plane strides
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     // Fragment bases of the transposed layers: layer 0 has ceil(enc / 32) x (W / 16)
@@ -216,7 +248,10 @@ __global__ void __launch_bounds__(256) k_train_bwd_mfma(const DevCfg c, const T*
                 finite = finite && finite_v<T>(bd[cc][kb]);
                 if (cc ? okb : oka) {
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) dyo[(size_t)(16 * kb + 8 * h + e) * n + q] = bd[cc][kb][e];
+                    for (int p = 0; p < NP; ++p)
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            dyo[p * pdo + (size_t)(16 * kb + 8 * h + e) * n + q] = mf::get<T>(bd[cc][kb], p, e);
                 }
             }
         mf::f32x16 acc[TH][2];
@@ -253,9 +288,12 @@ __global__ void __launch_bounds__(256) k_train_bwd_mfma(const DevCfg c, const T*
                     bg[cc][kb] = mf::pack8<T>(v);
                     finite = finite && finite_v<T>(bg[cc][kb]);
                     if (cc ? okb : oka) {
-                        T* dst = dyh + ((size_t)(l - 1) * W) * n + (cc ? qrb : qra);
+                        E* dst = dyh + ((size_t)(l - 1) * W) * n + (cc ? qrb : qra);
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) dst[(size_t)mf::chained_column(kb, h, e) * n] = bg[cc][kb][e];
+                        for (int p = 0; p < NP; ++p)
+#pragma unroll
+                            for (int e = 0; e < 8; ++e)
+                                dst[p * pdh + (size_t)mf::chained_column(kb, h, e) * n] = mf::get<T>(bg[cc][kb], p, e);
                     }
                 }
             if (l >= 2) {
@@ -313,12 +351,16 @@ template <class T>
 __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainMfmaWS ws, float* __restrict__ partials,
                                                     uint32_t n, uint32_t q_per_wave, uint32_t n_mlp) {
     using V = typename mf::Frag<T>::vec;
+    using E = typename mf::Op<T>::elem;
     constexpr int TI = 4;  // input tiles per pass
     const int l = blockIdx.x;
     const int H = c.n_layers - 1;
     const int iw = c.layer_in[l], ow = c.layer_out[l];
-    const T* X = l == 0 ? static_cast<const T*>(ws.x0) : static_cast<const T*>(ws.xh) + (size_t)(l - 1) * c.width * n;
-    const T* G = l == H ? static_cast<const T*>(ws.dyo) : static_cast<const T*>(ws.dyh) + (size_t)l * c.width * n;
+    const E* X = l == 0 ? static_cast<const E*>(ws.x0) : static_cast<const E*>(ws.xh) + (size_t)(l - 1) * c.width * n;
+    const E* G = l == H ? static_cast<const E*>(ws.dyo) : static_cast<const E*>(ws.dyh) + (size_t)l * c.width * n;
+    // plane strides of the split type: the buffers' sizes
+    const size_t px = (l == 0 ? (size_t)c.enc_padded : (size_t)H * c.width) * n;
+    const size_t pg = (l == H ? (size_t)c.n_out_pad : (size_t)H * c.width) * n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
     const uint32_t row = blockIdx.y * 4u + (uint32_t)wave;
@@ -336,8 +378,8 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < TI; ++b) acc[a][b] = mf::f32x16{};
-            const T* ga[2];
-            const T* xb[TI];
+            const E* ga[2];
+            const E* xb[TI];
 #pragma unroll
             for (int a = 0; a < 2; ++a) ga[a] = G + (size_t)(32 * (to0 + (a < nto ? a : 0)) + j) * n;
 #pragma unroll
@@ -351,25 +393,29 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     if (full) {
-                        av[a] = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(ga[a] + q + 8u * h));
+                        av[a] = mf::load_rows16<T>(ga[a] + q + 8u * h, pg);
                     } else {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            const uint32_t qq = q + 8u * h + e;
-                            av[a][e] = qq < qe ? ga[a][qq] : (T)0.0f;
-                        }
+                        for (int p = 0; p < mf::Op<T>::planes; ++p)
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t qq = q + 8u * h + e;
+                                mf::set<T>(av[a], p, e, qq < qe ? ga[a][p * pg + qq] : (E)0.0f);
+                            }
                     }
                 }
 #pragma unroll
                 for (int b = 0; b < TI; ++b) {
                     if (full) {
-                        bv[b] = __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(xb[b] + q + 8u * h));
+                        bv[b] = mf::load_rows16<T>(xb[b] + q + 8u * h, px);
                     } else {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            const uint32_t qq = q + 8u * h + e;
-                            bv[b][e] = qq < qe ? xb[b][qq] : (T)0.0f;
-                        }
+                        for (int p = 0; p < mf::Op<T>::planes; ++p)
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const uint32_t qq = q + 8u * h + e;
+                                mf::set<T>(bv[b], p, e, qq < qe ? xb[b][p * px + qq] : (E)0.0f);
+                            }
                     }
                 }
 #pragma unroll
@@ -403,8 +449,10 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
 // over input rows 32 t .. 32 t + 31 and output K-block kb; lane (h, i),
 // element e holds W_l[o][32 t + i] with o = 16 kb + 8 h + e on the output
 // layer and the chained permutation on hidden layers.
+// Split-BF16 operands: hi fragment 2 f, lo fragment 2 f + 1, as k_pack_frags.
 template <class T>
-__global__ void __launch_bounds__(256) k_pack_bfrags(const DevCfg c, const float* __restrict__ w, T* __restrict__ frag) {
+__global__ void __launch_bounds__(256) k_pack_bfrags(const DevCfg c, const float* __restrict__ w,
+                                                     typename mf::Op<T>::elem* __restrict__ frag) {
     const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     uint32_t base = 0;
     const int H = c.n_layers - 1;
@@ -420,7 +468,15 @@ __global__ void __launch_bounds__(256) k_pack_bfrags(const DevCfg c, const float
             const uint32_t hh = lane / 32u, i = lane % 32u;
             const uint32_t o = l == H ? 16u * kb + 8u * hh + e : (uint32_t)mf::chained_column((int)kb, (int)hh, (int)e);
             const uint32_t row = 32u * t + i;
-            frag[gid] = row < iw ? (T)w[c.layer_off[l] + o * iw + row] : (T)0.0f;
+            const float x = row < iw ? w[c.layer_off[l] + o * iw + row] : 0.0f;
+            if constexpr (mf::Op<T>::planes == 2) {
+                const size_t g0 = (size_t)(gid / 512u) * 1024u + gid % 512u;
+                const __bf16 hi = (__bf16)x;
+                frag[g0] = hi;
+                frag[g0 + 512u] = (__bf16)(x - (float)hi);
+            } else {
+                frag[gid] = (T)x;
+            }
             return;
         }
         base += cnt;
@@ -450,14 +506,15 @@ __host__ uint32_t n_bfrag_of(const DevCfg& c) {
 template <class T, int W, int NOUT, int H>
 hipError_t launch_fwd_t(const DevCfg& c, const void* wfrag, const void* table, const void* genc, const float* in,
                         const TrainMfmaWS& ws, uint32_t n, int n_cu, hipStream_t s) {
+    using G = typename mf::Op<T>::table;
     const uint32_t n_frag = n_mlp_frags(c);
-    const size_t lds = (size_t)n_frag * 1024u;
+    const size_t lds = (size_t)n_frag * 1024u * mf::Op<T>::planes;
     auto kern = k_train_fwd_mfma<T, W, NOUT, H>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const uint32_t grid = persistent_grid((const void*)kern, lds, n, n_cu);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, c, static_cast<const T*>(wfrag),
-                       static_cast<const T*>(table), static_cast<const T*>(genc), in, ws, n, n_frag);
+                       static_cast<const G*>(table), static_cast<const G*>(genc), in, ws, n, n_frag);
     return hipGetLastError();
 }
 
@@ -465,7 +522,7 @@ template <class T, int W, int NOUT, int H>
 hipError_t launch_bwd_t(const DevCfg& c, const void* bfrag, const float* dout, const TrainMfmaWS& ws, float* denc,
                         uint32_t n, uint32_t* overflow, int n_cu, hipStream_t s) {
     const uint32_t n_bfrag = n_bfrag_of(c);
-    const size_t lds = (size_t)n_bfrag * 1024u;
+    const size_t lds = (size_t)n_bfrag * 1024u * mf::Op<T>::planes;
     auto kern = k_train_bwd_mfma<T, W, NOUT, H>;
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -506,6 +563,15 @@ bool mfma_train_supported(const DevCfg& c) {
     CASE(TT, 128, 64, 3)            \
     CASE(TT, 64, 96, 3)             \
     CASE(TT, 128, 96, 3)
+// split-BF16 operands: width 64 (two weight planes of a 128-wide network exceed the LDS)
+#define NPMG_TRAIN_SHAPES_NARROW(CASE, TT) \
+    CASE(TT, 64, 32, 2)                    \
+    CASE(TT, 64, 32, 3)                    \
+    CASE(TT, 64, 32, 4)                    \
+    CASE(TT, 64, 64, 2)                    \
+    CASE(TT, 64, 64, 3)                    \
+    CASE(TT, 64, 64, 4)                    \
+    CASE(TT, 64, 96, 3)
 
 hipError_t launch_train_forward_mfma(const DevCfg& c, int precision, const void* wfrag, const void* table, void* genc,
                                      const float* in, const TrainMfmaWS& ws, uint32_t n, int n_cu, hipStream_t s) {
@@ -523,6 +589,7 @@ hipError_t launch_train_forward_mfma(const DevCfg& c, int precision, const void*
         return launch_fwd_t<TT, WW, NO, HH>(c, wfrag, table, genc, in, ws, n, n_cu, s);
     if (precision == PREC_BF16) { NPMG_TRAIN_SHAPES(NPMG_FWD_CASE, __bf16) }
     else if (precision == PREC_FP16) { NPMG_TRAIN_SHAPES(NPMG_FWD_CASE, _Float16) }
+    else if (precision == PREC_BF16X3) { NPMG_TRAIN_SHAPES_NARROW(NPMG_FWD_CASE, mf::Bf16x3) }
 #undef NPMG_FWD_CASE
     return hipErrorInvalidValue;
 }
@@ -537,10 +604,12 @@ hipError_t launch_train_backward_mfma(const DevCfg& c, int precision, const void
         return launch_bwd_t<TT, WW, NO, HH>(c, bfrag, dout, ws, denc, n, overflow, n_cu, s);
     if (precision == PREC_BF16) { NPMG_TRAIN_SHAPES(NPMG_BWD_CASE, __bf16) }
     else if (precision == PREC_FP16) { NPMG_TRAIN_SHAPES(NPMG_BWD_CASE, _Float16) }
+    else if (precision == PREC_BF16X3) { NPMG_TRAIN_SHAPES_NARROW(NPMG_BWD_CASE, mf::Bf16x3) }
 #undef NPMG_BWD_CASE
     return hipErrorInvalidValue;
 }
 #undef NPMG_TRAIN_SHAPES
+#undef NPMG_TRAIN_SHAPES_NARROW
 
 size_t train_mask_words(const DevCfg& c, uint32_t n) {
     return (size_t)((n + 63u) / 64u) * (size_t)c.n_hidden * (size_t)(c.width / 32) * 64u;
@@ -571,6 +640,8 @@ hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& 
     const dim3 grid(c.n_layers, rows / 4u);
     if (precision == PREC_BF16) hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
     else if (precision == PREC_FP16) hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
+    else if (precision == PREC_BF16X3)
+        hipLaunchKernelGGL((k_wgrad_mfma<mf::Bf16x3>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
     else return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -591,6 +662,8 @@ hipError_t launch_pack_bfrags(const DevCfg& c, int precision, const float* w, vo
     const dim3 grid((n + 255) / 256);
     if (precision == PREC_BF16) hipLaunchKernelGGL((k_pack_bfrags<__bf16>), grid, dim3(256), 0, s, c, w, (__bf16*)frag);
     else if (precision == PREC_FP16) hipLaunchKernelGGL((k_pack_bfrags<_Float16>), grid, dim3(256), 0, s, c, w, (_Float16*)frag);
+    else if (precision == PREC_BF16X3)
+        hipLaunchKernelGGL((k_pack_bfrags<mf::Bf16x3>), grid, dim3(256), 0, s, c, w, (__bf16*)frag);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -28,6 +28,16 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
+// Split-BF16 operand ("precision": "bf16x3"): every value x is carried as
+// hi = bf16(x) and lo = bf16(x - hi), about 16 significant bits, and each
+// product of two operands is formed from three BF16 products, hi*hi + hi*lo
+// + lo*hi (the lo*lo term is below the FP32 accumulator's rounding). A
+// fragment is two planes: the hi fragment, then the lo fragment.
+struct Bf16x3 {};
+struct bf16x8x2 {
+    bf16x8 hi, lo;
+};
+
 // FP8 E4M3 operand byte (inference only). v_mfma_f32_32x32x16_fp8_fp8 has the
 // fragment layout of the 16-bit types above with 8 one-byte values per lane.
 struct Fp8 {
@@ -54,6 +64,68 @@ template <> struct Frag<_Float16> {
     }
 };
 
+template <> struct Frag<Bf16x3> {
+    using vec = bf16x8x2;
+    __device__ static __forceinline__ f32x16 mma(const vec& a, const vec& b, f32x16 c) {
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.lo, b.hi, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.hi, b.lo, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.hi, b.hi, c, 0, 0, 0);
+    }
+};
+
+// Storage of an operand type: element type and planes (hi, lo) per value, and
+// the type of the grid table and level-major features that feed it.
+template <class O> struct Op {
+    using elem = O;
+    using table = O;
+    static constexpr int planes = 1;
+};
+template <> struct Op<Bf16x3> {
+    using elem = __bf16;
+    using table = _Float16;
+    static constexpr int planes = 2;
+};
+template <> struct Op<Fp8> {
+    using elem = uint8_t;
+    using table = __bf16;
+    static constexpr int planes = 1;
+};
+
+// Element e of plane p of a fragment value, as stored.
+template <class O>
+__device__ __forceinline__ typename Op<O>::elem get(const typename Frag<O>::vec& v, int p, int e) {
+    if constexpr (Op<O>::planes == 2) return p ? v.lo[e] : v.hi[e];
+    else return v[e];
+}
+template <class O>
+__device__ __forceinline__ void set(typename Frag<O>::vec& v, int p, int e, typename Op<O>::elem x) {
+    if constexpr (Op<O>::planes == 2) {
+        if (p) v.lo[e] = x;
+        else v.hi[e] = x;
+    } else {
+        v[e] = x;
+    }
+}
+// Value of element e as a float (hi + lo for the split type).
+template <class O>
+__device__ __forceinline__ float value(const typename Frag<O>::vec& v, int e) {
+    if constexpr (Op<O>::planes == 2) return (float)v.hi[e] + (float)v.lo[e];
+    else return (float)v[e];
+}
+// 8 consecutive 16-bit elements of each plane (16-byte loads; p0 + plane_stride
+// is the lo plane of the split type).
+template <class O>
+__device__ __forceinline__ typename Frag<O>::vec load_rows16(const typename Op<O>::elem* p0, size_t plane_stride) {
+    typename Frag<O>::vec v;
+    if constexpr (Op<O>::planes == 2) {
+        v.hi = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p0));
+        v.lo = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p0 + plane_stride));
+    } else {
+        v = __builtin_bit_cast(typename Frag<O>::vec, *reinterpret_cast<const uint4*>(p0));
+    }
+    return v;
+}
+
 template <> struct Frag<Fp8> {
     using vec = long;  // 8 E4M3 values, element e in byte e
     __device__ static __forceinline__ f32x16 mma(vec a, vec b, f32x16 c) {
@@ -65,7 +137,15 @@ template <> struct Frag<Fp8> {
 // within +-kFp8Max, the callers clamp).
 template <class T>
 __device__ __forceinline__ typename Frag<T>::vec pack8(const float* v) {
-    if constexpr (sizeof(typename Frag<T>::vec) == 8) {
+    if constexpr (Op<T>::planes == 2) {
+        bf16x8x2 r;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            r.hi[e] = (__bf16)v[e];
+            r.lo[e] = (__bf16)(v[e] - (float)r.hi[e]);
+        }
+        return r;
+    } else if constexpr (sizeof(typename Frag<T>::vec) == 8) {
         int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
         lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
         int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
@@ -88,7 +168,11 @@ __device__ __forceinline__ uint8_t to_fp8(float x) {
 // contiguous block (16 bytes per lane, or 8 for FP8): conflict-free LDS reads.
 template <class T>
 __device__ __forceinline__ typename Frag<T>::vec load_frag(const T* base, int frag, int lane) {
-    if constexpr (sizeof(typename Frag<T>::vec) == 8) {
+    if constexpr (Op<T>::planes == 2) {
+        const uint4* p = reinterpret_cast<const uint4*>(base);
+        return bf16x8x2{__builtin_bit_cast(bf16x8, p[(2 * frag) * 64 + lane]),
+                        __builtin_bit_cast(bf16x8, p[(2 * frag + 1) * 64 + lane])};
+    } else if constexpr (sizeof(typename Frag<T>::vec) == 8) {
         const uint2 raw = reinterpret_cast<const uint2*>(base)[frag * 64 + lane];
         return __builtin_bit_cast(long, raw);
     } else {
@@ -113,6 +197,20 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
 // 1/2 - S(1 - t), with S(u) = 15/16 (u - 2u^3/3 + u^5/5) the odd part of the
 // kernel's CDF. Bins b, b + 1 are the 32-bit window at 16-bit slot
 // e = (b - kx + 2) mod 16 of the slot sequence (0, m[kx-1], m[kx], m[kx+1], 0, ...).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// The 8 16-bit slots of bins 8h .. 8h+7 taken from the slot sequence `slots`.
+__device__ __forceinline__ u32x4 oneblob16_place(uint64_t slots, int kx, int h) {
+    const int base = 8 * h + 2 - kx;
+    u32x4 d;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t e = (uint32_t)(base + 2 * j) & 15u;
+        d[j] = e < 4u ? (uint32_t)(slots >> (16u * e)) : 0u;
+    }
+    return d;
+}
+
 template <class T>
 __device__ __forceinline__ typename mf::Frag<T>::vec oneblob16_frag(float x, int h) {
     const float s = x * 16.0f;
@@ -124,16 +222,18 @@ __device__ __forceinline__ typename mf::Frag<T>::vec oneblob16_frag(float x, int
         return u * __builtin_fmaf(u2, __builtin_fmaf(u2, 0.1875f, -0.625f), 0.9375f);
     };
     const float st = odd(t), sr = odd(1.0f - t);
-    const uint64_t slots = ((uint64_t)pack2<T>(st + sr, 0.5f - sr) << 32) | (uint64_t)pack2<T>(0.0f, 0.5f - st);
-    const int base = 8 * h + 2 - kx;
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 d;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t e = (uint32_t)(base + 2 * j) & 15u;
-        d[j] = e < 4u ? (uint32_t)(slots >> (16u * e)) : 0u;
+    const float mp = 0.5f - st, mc = st + sr, mn = 0.5f - sr;
+    if constexpr (Op<T>::planes == 2) {
+        const __bf16 hp = (__bf16)mp, hc = (__bf16)mc, hn = (__bf16)mn;
+        const uint64_t sh = ((uint64_t)pack2<__bf16>(mc, mn) << 32) | (uint64_t)pack2<__bf16>(0.0f, mp);
+        const uint64_t sl = ((uint64_t)pack2<__bf16>(mc - (float)hc, mn - (float)hn) << 32) |
+                            (uint64_t)pack2<__bf16>(0.0f, mp - (float)hp);
+        return bf16x8x2{__builtin_bit_cast(bf16x8, oneblob16_place(sh, kx, h)),
+                        __builtin_bit_cast(bf16x8, oneblob16_place(sl, kx, h))};
+    } else {
+        const uint64_t slots = ((uint64_t)pack2<T>(mc, mn) << 32) | (uint64_t)pack2<T>(0.0f, mp);
+        return __builtin_bit_cast(typename mf::Frag<T>::vec, oneblob16_place(slots, kx, h));
     }
-    return __builtin_bit_cast(typename mf::Frag<T>::vec, d);
 }
 
 }  // namespace mf

@@ -29,19 +29,33 @@ using namespace dev;
 // Layer 0 of the "grid, then OneBlob(16)" layout (DevCfg::ob_nkb > 0) for a
 // 16-bit operand type: the grid K-blocks straight from the level-major
 // features, then one K-block of mf::oneblob16_frag bins per OneBlob input dim.
-template <class T, int TH>
-__device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, const T* __restrict__ genc,
+// G is the type of the level-major features, T the operand type: the same
+// 16-bit type, or the split BF16 operand built from FP16 features (exact: an
+// FP16 value has 11 significant bits, hi and lo hold 16).
+template <class G, class T, int TH>
+__device__ __forceinline__ void layer0_oneblob16(const DevCfg& c, const T* sw, const G* __restrict__ genc,
                                                  const QueryIO& io, uint32_t n, uint32_t qa, uint32_t qb, int lane,
                                                  mf::f32x16 (&acc)[TH][2]) {
     using V = typename mf::Frag<T>::vec;
     const int h = lane >> 5, KB0 = c.enc_padded / 16;
     for (int kb = 0; kb < c.ob_kb0; ++kb) {
         V ba, bb;
-        const T* src = genc + (size_t)(16 * kb + 8 * h) * n;
+        const G* src = genc + (size_t)(16 * kb + 8 * h) * n;
+        if constexpr (std::is_same_v<G, T>) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            ba[i] = src[(size_t)i * n + qa];
-            bb[i] = src[(size_t)i * n + qb];
+            for (int i = 0; i < 8; ++i) {
+                ba[i] = src[(size_t)i * n + qa];
+                bb[i] = src[(size_t)i * n + qb];
+            }
+        } else {
+            float va[8], vb[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                va[i] = (float)src[(size_t)i * n + qa];
+                vb[i] = (float)src[(size_t)i * n + qb];
+            }
+            ba = mf::pack8<T>(va);
+            bb = mf::pack8<T>(vb);
         }
 #pragma unroll
         for (int t = 0; t < TH; ++t) {
@@ -141,7 +155,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
                                                 uint32_t q0, uint32_t q_end, int lane,
                                                 const float* __restrict__ qscale) {
     using V = typename mf::Frag<O>::vec;
-    constexpr bool F8 = sizeof(V) == 8;
+    constexpr bool F8 = std::is_same_v<O, mf::Fp8>;
     constexpr int NOUT = (P * K + 15) / 16 * 16;
     static_assert(NOUT % 32 == 0, "whole 32-row output tiles");
     constexpr int TO = NOUT / 32;  // output tiles
@@ -158,7 +172,7 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
     if constexpr (FAST) {
         if (c.ob_nkb > 0 && genc) {
             if constexpr (F8) layer0_oneblob16_fp8<T, TH>(c, sw, genc, io, n, qa, qb, lane, qscale[0], acc);
-            else layer0_oneblob16<O, TH>(c, sw, genc, io, n, qa, qb, lane, acc);
+            else layer0_oneblob16<T, O, TH>(c, sw, genc, io, n, qa, qb, lane, acc);
             layer0_done = true;
         }
     }

@@ -20,11 +20,33 @@ def _np(t):
 
 
 def _round(a, prec):
+    """An operand rounded as the kernels round it: FP16, BF16, or for bf16x3
+    the split pair hi = bf16(x), lo = bf16(x - hi) of the FP32 value."""
     import torch
 
     if prec == "fp16":
-        return a.astype(np.float16).astype(np.float64)
-    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+        return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+    b = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    if prec == "bf16x3":
+        x = np.asarray(a, np.float32)
+        hi = b(x)
+        return hi + b(x - hi.astype(np.float32))
+    return b(a)
+
+
+def _storage(prec):
+    """Rounding of the grid table and of the level-major grid features: FP16
+    for bf16x3 (then split exactly), else the operand type."""
+    return "fp16" if prec == "bf16x3" else prec
+
+
+def _round_enc(enc, prec, oc):
+    """Layer-0 inputs: the grid rows (the first n_levels x F) in the storage
+    type, then every row as an operand."""
+    e = np.array(enc, np.float64)
+    g = oc.n_levels * oc.n_features
+    e[:g] = _round(e[:g], _storage(prec))
+    return _round(e, prec)
 
 
 def _configs(prec):
@@ -41,13 +63,17 @@ def _configs(prec):
     }
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3"])
 @pytest.mark.parametrize("name", ["small", "full", "reference", "k16", "w128"])
 def test_mfma_query(gpu, oracle, prec, name):
     import npmg
     import pyoracle
 
     cfg = _configs(prec)[name]
+    if prec == "bf16x3" and name == "w128":
+        with pytest.raises(npmg.NpmgError, match="width 64"):
+            npmg.GuidingNetwork(cfg, seed=7272)
+        return
     net = npmg.GuidingNetwork(cfg, seed=7272)
     oc = pyoracle.make_cfg(cfg)
     n = 4096
@@ -57,9 +83,9 @@ def test_mfma_query(gpu, oracle, prec, name):
     p = _np(net.get_params(1)).astype(np.float32)
     n_mlp = net.n_mlp
     # emulation of the rounding points
-    table = _round(p[n_mlp:], prec).astype(np.float32)
+    table = _round(p[n_mlp:], _storage(prec)).astype(np.float32)
     enc = oracle.encode(oc, table, h["inputs"])  # FP32 features, as the kernel computes them
-    a = _round(enc, prec)
+    a = _round_enc(enc, prec, oc)
     layers = []
     off, n_in = 0, net.encoded_width
     for l in range(oc.n_hidden_layers + 1):
@@ -87,5 +113,5 @@ def test_mfma_query(gpu, oracle, prec, name):
     # end to end against the FP64 oracle with unrounded parameters (cost of the operand precision)
     gold = oracle.query(oc, p, h["inputs"], h["normal"], h["u"], h["eval_dirs"], dtype=np.float64)
     rel = np.abs(_np(out["eval_pdf"]) - gold["eval_pdf"]) / gold["eval_pdf"]
-    assert np.median(rel) < (2e-2 if prec == "bf16" else 3e-3), np.median(rel)
+    assert np.median(rel) < {"bf16": 2e-2, "fp16": 3e-3, "bf16x3": 1e-3}[prec], np.median(rel)
     net.close()

@@ -10,6 +10,9 @@ The backward chain and weight gradients are checked three ways:
     cost of the operand precision, bounded by 1e-2 relative;
   * against FP64 with its own masks, which adds the units whose
     pre-activation rounds across zero: reported, with a loose bound.
+Samples with a hidden pre-activation within 1e-5 of zero (relative to the
+layer) get no output gradient in the emulation comparisons: their ReLU sign in
+the kernel depends on its FP32 summation order.
 Dynamic loss scaling is driven from the host: targets scaled by 1e6 overflow
 the FP16 operands, which must skip the step and halve the scale.
 """
@@ -25,11 +28,33 @@ def _np(t):
 
 
 def _round(a, prec):
+    """An operand rounded as the kernels round it: FP16, BF16, or for bf16x3
+    the split pair hi = bf16(x), lo = bf16(x - hi) of the FP32 value."""
     import torch
 
     if prec == "fp16":
         return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
-    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    b = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    if prec == "bf16x3":
+        x = np.asarray(a, np.float32)
+        hi = b(x)
+        return hi + b(x - hi.astype(np.float32))
+    return b(a)
+
+
+def _storage(prec):
+    """Rounding of the grid table and of the level-major grid features: FP16
+    for bf16x3 (then split exactly), else the operand type."""
+    return "fp16" if prec == "bf16x3" else prec
+
+
+def _round_enc(enc, prec, oc):
+    """Layer-0 inputs: the grid rows (the first n_levels x F) in the storage
+    type, then every row as an operand."""
+    e = np.array(enc, np.float64)
+    g = oc.n_levels * oc.n_features
+    e[:g] = _round(e[:g], _storage(prec))
+    return _round(e, prec)
 
 
 def _configs(prec):
@@ -63,18 +88,32 @@ def _chain(oracle, oc, net, p, inp, prec=None, masks=None):
     rnd = (lambda a: _round(a, prec)) if prec else (lambda a: np.asarray(a, np.float64))
     table = p[net.n_mlp:]
     if prec:
-        enc = oracle.encode(oc, rnd(table).astype(np.float32), inp)
+        enc = oracle.encode(oc, _round(table, _storage(prec)).astype(np.float32), inp)
+        x0 = _round_enc(enc, prec, oc)
     else:
         enc = oracle.encode(oc, table, inp, dtype=np.float64)
+        x0 = enc
     W = [rnd(w) for w in _layers(oc, p, net.encoded_width, net.padded_output)]
-    xs, ms = [rnd(enc)], []
+    xs, ms, zs = [x0], [], []
     for l, w in enumerate(W):
         z = w @ xs[-1]
         if l < len(W) - 1:
             m = (z > 0) if masks is None else masks[l]
             ms.append(m)
+            zs.append(z)
             xs.append(rnd(np.where(m, z, 0.0)))
-    return {"W": W, "xs": xs, "masks": ms, "raw": z, "rnd": rnd}
+    return {"W": W, "xs": xs, "masks": ms, "raw": z, "rnd": rnd, "zs": zs}
+
+
+def _ambiguous(oc, net, p, inp, prec, oracle, rel=1e-5):
+    """Samples with a hidden pre-activation within rel x the layer's largest of
+    zero in the emulation: there the kernel's ReLU sign depends on its FP32
+    summation order, which the emulation does not reproduce."""
+    emu = _chain(oracle, oc, net, p, inp, prec)
+    bad = np.zeros(inp.shape[1], bool)
+    for z in emu["zs"]:
+        bad |= np.any(np.abs(z) <= rel * np.abs(z).max(), axis=0)
+    return bad
 
 
 def _backward(ch, dout):
@@ -98,10 +137,10 @@ def _rel2(a, b):
 # network's own ReLU signs: units whose pre-activation is within rounding of
 # zero switch sides, each flip contributing a whole term, so this error grows
 # like sqrt(eps) (measured: bf16 4-8e-2, fp16 1.5-3e-2 for random dL/draw).
-FLIP_BOUND = {"bf16": 0.15, "fp16": 0.06}
+FLIP_BOUND = {"bf16": 0.15, "fp16": 0.06, "bf16x3": 0.02}
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3"])
 @pytest.mark.parametrize("name", ["small", "full", "reference", "k16", "w128"])
 def test_mfma_backward(gpu, oracle, prec, name):
     import torch
@@ -109,16 +148,19 @@ def test_mfma_backward(gpu, oracle, prec, name):
     import pyoracle
 
     cfg = _configs(prec)[name]
+    if prec == "bf16x3" and name == "w128":
+        pytest.skip("bf16x3 networks are 64 wide (two weight planes of a 128-wide network exceed the LDS)")
     net = npmg.GuidingNetwork(cfg, seed=7272)
     oc = pyoracle.make_cfg(cfg)
     n = 2000  # not a multiple of 64: partial last tile
     q = npmg.synth_queries(8, n)
     rng = np.random.default_rng(3)
     dout = rng.normal(size=(net.padded_output, n)).astype(np.float32)
-    dparams, denc = net.backward(q["inputs"], torch.from_numpy(dout).cuda(), want_denc=True)
-    g = _np(dparams)
     p = _np(net.get_params(0))
     inp = _np(q["inputs"])
+    dout[:, _ambiguous(oc, net, p, inp, prec, oracle)] = 0.0  # those samples' gradients stay out
+    dparams, denc = net.backward(q["inputs"], torch.from_numpy(dout).cuda(), want_denc=True)
+    g = _np(dparams)
     # 1) the kernels' own rounding points: only accumulation order remains
     emu = _chain(oracle, oc, net, p, inp, prec)
     dw_e, denc_e = _backward(emu, dout)
@@ -142,7 +184,7 @@ def test_mfma_backward(gpu, oracle, prec, name):
     net.close()
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3"])
 def test_mfma_backward_fast_oneblob(gpu, oracle, prec):
     """With the fast numerics the training forward builds the OneBlob fragments
     from the three non-zero bins of the quartic kernel, as the fused query
@@ -162,10 +204,11 @@ def test_mfma_backward_fast_oneblob(gpu, oracle, prec):
     q = npmg.synth_queries(8, n)
     rng = np.random.default_rng(3)
     dout = rng.normal(size=(net.padded_output, n)).astype(np.float32)
-    dparams, _ = net.backward(q["inputs"], torch.from_numpy(dout).cuda())
-    g = _np(dparams)
     p = _np(net.get_params(0))
     inp = _np(q["inputs"])
+    dout[:, _ambiguous(oc, net, p, inp, prec, oracle)] = 0.0
+    dparams, _ = net.backward(q["inputs"], torch.from_numpy(dout).cuda())
+    g = _np(dparams)
     emu = _chain(oracle, oc, net, p, inp, prec)
     dw_e, denc_e = _backward(emu, dout)
     e_w, e_grid = _rel2(g[:net.n_mlp], dw_e), _rel2(g[net.n_mlp:], oracle.grid_backward(oc, inp, denc_e))
@@ -184,7 +227,7 @@ def test_mfma_backward_fast_oneblob(gpu, oracle, prec):
     net.close()
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3"])
 def test_mfma_train_step_gradient(gpu, oracle, prec):
     """The train step's loss and gradient (adjoint + backward) against FP64."""
     import npmg
@@ -222,7 +265,7 @@ def test_mfma_train_step_gradient(gpu, oracle, prec):
     net.close()
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3"])
 def test_mfma_training_converges(gpu, prec):
     """Training on the analytic field lowers the loss like the FP32 path does."""
     import npmg
