@@ -298,19 +298,22 @@ __device__ __forceinline__ void query_tile_mfma(const DevCfg& c, const O* sw, co
 
 // Waves per workgroup: the weights of a 128-wide network fill most of a CU's
 // LDS (120 KB in BF16), so one workgroup per CU is all that fits; 8 waves
-// then put two waves on each SIMD to hide the MFMA and memory latencies.
-template <int W>
-constexpr int query_waves() { return W >= 128 ? 8 : 4; }
+// then put two waves on each SIMD to hide the MFMA and memory latencies. The
+// two weight planes of the split BF16 operands (72 KB at width 64) get 8 waves
+// as well, so that one workgroup per CU leaves LDS for the grid encode that
+// runs beside it in the pipelined query path.
+template <int W, class O>
+constexpr int query_waves() { return W >= 128 || mf::Op<O>::planes == 2 ? 8 : 4; }
 
 // Queries [q_begin, q_end) of SoA arrays with row stride n.
 template <class T, class O, int W, int K, int P, bool FAST>
-__global__ void __launch_bounds__(64 * query_waves<W>()) k_query_mfma(const DevCfg c, const O* __restrict__ wfrag,
+__global__ void __launch_bounds__((64 * query_waves<W, O>())) k_query_mfma(const DevCfg c, const O* __restrict__ wfrag,
                                                     const T* __restrict__ table, const T* __restrict__ genc,
                                                     const QueryIO io, uint32_t n, uint32_t n_frag, uint32_t q_begin,
                                                     uint32_t q_end, const float* __restrict__ qscale) {
     extern __shared__ uint4 smem_frag[];
     constexpr uint32_t u4_per_frag = 64u * sizeof(typename mf::Frag<O>::vec) / 16u;
-    constexpr uint32_t NW = query_waves<W>();
+    constexpr uint32_t NW = query_waves<W, O>();
     for (uint32_t i = threadIdx.x; i < n_frag * u4_per_frag; i += 64u * NW)
         smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
     __syncthreads();
@@ -328,12 +331,12 @@ __global__ void __launch_bounds__(64 * query_waves<W>()) k_query_mfma(const DevC
 constexpr uint32_t kTileBatch = 4;
 
 template <class T, class O, int W, int K, int P, bool FAST>
-__global__ void __launch_bounds__(64 * query_waves<W>()) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
+__global__ void __launch_bounds__((64 * query_waves<W, O>())) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
                                                            const T* __restrict__ table, const QueueSet qs,
                                                            uint32_t n_frag, const float* __restrict__ qscale) {
     extern __shared__ uint4 smem_frag[];
     constexpr uint32_t u4_per_frag = 64u * sizeof(typename mf::Frag<O>::vec) / 16u;
-    for (uint32_t i = threadIdx.x; i < n_frag * u4_per_frag; i += 64u * query_waves<W>())
+    for (uint32_t i = threadIdx.x; i < n_frag * u4_per_frag; i += 64u * query_waves<W, O>())
         smem_frag[i] = reinterpret_cast<const uint4*>(wfrag)[i];
     __syncthreads();
     const O* sw = reinterpret_cast<const O*>(smem_frag);
@@ -376,12 +379,12 @@ hipError_t launch_mfma_t(const DevCfg& c, const void* wfrag, const void* table, 
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * query_waves<W>(), lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * query_waves<W, O>(), lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     if (bpc > 0 && bpc < per_cu) per_cu = bpc;
     const uint32_t tiles = (q_end - q_begin + 63u) / 64u;
-    constexpr uint32_t NW = query_waves<W>();
+    constexpr uint32_t NW = query_waves<W, O>();
     const uint32_t want = (tiles + NW - 1u) / NW;
     const uint32_t grid = want < (uint32_t)(n_cu * per_cu) ? want : (uint32_t)(n_cu * per_cu);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, s, c, static_cast<const O*>(wfrag),
@@ -399,11 +402,11 @@ hipError_t launch_mfma_queues_t(const DevCfg& c, const void* wfrag, const void* 
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * query_waves<W>(), lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * query_waves<W, O>(), lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     if (blocks_per_cu > 0 && blocks_per_cu < per_cu) per_cu = blocks_per_cu;
-    hipLaunchKernelGGL(kern, dim3((uint32_t)(n_cu * per_cu)), dim3(64 * query_waves<W>()), lds, s, c,
+    hipLaunchKernelGGL(kern, dim3((uint32_t)(n_cu * per_cu)), dim3(64 * query_waves<W, O>()), lds, s, c,
                        static_cast<const O*>(wfrag),
                        static_cast<const T*>(table), qs, n_frag, qscale);
     return hipGetLastError();
