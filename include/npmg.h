@@ -154,6 +154,21 @@ typedef struct npmg_queue {
     float* out_selection; /* [capacity] learned selection probability (mixture.learn_selection), or NULL */
 } npmg_queue;
 
+/* npmg_sample_pdf that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). No counterpart in the reference. The level-major grid
+ * features of the MFMA path go to a 16 MiB device array that the library
+ * holds on each GPU for the life of the process (n x n_levels x
+ * features_per_level x 2 bytes at most: 262144 queries of a 16-level,
+ * 2-feature grid; larger n is an error). No timing events are recorded, and
+ * an FP8 network must be calibrated beforehand (npmg_fp8_calibrate). Every
+ * call on one GPU shares that GPU's array and overwrites it: calls that
+ * overlap on one GPU (other streams, other contexts, other threads) race. */
+npmg_status npmg_sample_pdf_noalloc(npmg_ctx* ctx, const float* in, const float* normal, const float* u,
+                                    const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                                    float* out_eval_pdf, uint32_t* out_lobe, float* raw_out, uint32_t n,
+                                    uint32_t flags, npmg_stream stream);
+
 /* npmg_sample_pdf with the outputs described by one npmg_queue (row stride n,
  * its capacity is ignored), including out_selection. */
 npmg_status npmg_sample_pdf_ex(npmg_ctx* ctx, const npmg_queue* q, uint32_t n, uint32_t n_eval, uint32_t flags,

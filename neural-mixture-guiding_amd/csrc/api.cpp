@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -161,6 +162,32 @@ void need(bool ok, const char* msg) {
 
 }  // namespace
 
+// Scratch of npmg_sample_pdf_noalloc: the level-major grid features (16-bit,
+// [L F][n]) of the MFMA query path in a file-scope device array, so that the
+// call allocates nothing. Each GPU holds its own copy for the life of the
+// process; its address is looked up once per device, with that device
+// current, when the first context on it is created (scratch_on_device).
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on device dev (current), looked up on first use.
+void* scratch_on_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
 // Unused dynamic LDS per grid-encode workgroup in the pipelined query path
 // (encode of chunk c + 1 beside the fused kernel of chunk c): caps the encode
 // at 5 workgroups per CU so that the fused kernel's workgroups find room.
@@ -204,6 +231,7 @@ struct npmg_ctx {
     uint32_t tune_chunks = 4;
     int tune_bpc = 0;
     uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
+    void* noalloc_scratch = nullptr;    // this device's `scratch` (npmg_sample_pdf_noalloc)
     uint32_t* pack_scratch = nullptr;   // block totals of the training packer
     size_t pack_cap = 0;
     float* stage = nullptr;  // contiguous batch of npmg_train_records: in [9], dir [2], rgb [3], pdf, bsdf pdf
@@ -342,35 +370,47 @@ struct npmg_ctx {
     }
 
     // Fused query with the training (0) or inference (1) parameters in the configured precision.
-    void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s) {
+    // genc_buf: a caller-provided buffer for the level-major grid features (at least
+    // L F n 16-bit values); then the call allocates nothing: no workspace growth, no
+    // FP8 calibration (an uncalibrated FP8 network is an error) and no timing events.
+    void query(const QueryIO& io, uint32_t n, bool inference, hipStream_t s, void* genc_buf = nullptr) {
+        const bool no_alloc = genc_buf != nullptr;
+        auto run = [&](int tag, hipStream_t st, auto&& f) {
+            if (no_alloc) f();
+            else timed(tag, st, f);
+        };
         if (half()) {
             const void* frag = inference ? frag_infer : frag_train;
             const void* tab = inference ? htab_infer : htab_train;
             const float* q8 = nullptr;
             if (inference && cfg.infer_fp8) {
-                if (!fp8_calibrated && n) calibrate_fp8(io.in, n, std::min(n, kFp8CalibMax), s);
+                if (!fp8_calibrated && n) {
+                    if (no_alloc) throw StateError("FP8 inference is not calibrated: call npmg_fp8_calibrate first");
+                    calibrate_fp8(io.in, n, std::min(n, kFp8CalibMax), s);
+                }
                 q8 = fp8_scales + 2 * cfg.d.n_layers;
             }
             if (!mfma_split_grid(cfg.d)) {
-                timed(NPMG_TIME_QUERY, s, [&] {
+                run(NPMG_TIME_QUERY, s, [&] {
                     hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, nullptr, io, n, 0u, n, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
                 return;
             }
-            ensure_genc(n);
+            if (!no_alloc) ensure_genc(n);
+            void* g = no_alloc ? genc_buf : genc;
             // Large batches: tune_chunks (<= kPipe) chunks; the encodes run on aux (after everything
-            // already queued on s, which may still read genc), chunk c's fused
+            // already queued on s, which may still read g), chunk c's fused
             // kernel on s after chunk c's encode.
             const uint32_t chunks = n >= (1u << 18) ? tune_chunks : 1u;
             const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
             if (chunks == 1u) {
-                timed(NPMG_TIME_GRID_ENCODE, s, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, 0u, n, s, 0u),
+                run(NPMG_TIME_GRID_ENCODE, s, [&] {
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, 0u, n, s, 0u),
                               "grid encode");
                 });
-                timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, 0u, n, n_cu, tune_bpc, q8, s),
+                run(NPMG_TIME_QUERY, s, [&] {
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, g, io, n, 0u, n, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
                 return;
@@ -379,8 +419,8 @@ struct npmg_ctx {
             hip_check(hipStreamWaitEvent(aux, ev_fork, 0), "stream wait");
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
-                timed(NPMG_TIME_GRID_ENCODE, aux, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, genc, n, b, e, aux, kEncodeLdsReserve),
+                run(NPMG_TIME_GRID_ENCODE, aux, [&] {
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, b, e, aux, kEncodeLdsReserve),
                               "grid encode");
                 });
                 hip_check(hipEventRecord(ev_chunk[k], aux), "event");
@@ -388,14 +428,14 @@ struct npmg_ctx {
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 hip_check(hipStreamWaitEvent(s, ev_chunk[k], 0), "stream wait");
-                timed(NPMG_TIME_QUERY, s, [&] {
-                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, genc, io, n, b, e, n_cu, tune_bpc, q8, s),
+                run(NPMG_TIME_QUERY, s, [&] {
+                    hip_check(launch_query_mfma(cfg.d, cfg.precision, frag, tab, g, io, n, b, e, n_cu, tune_bpc, q8, s),
                               "query (mfma)");
                 });
             }
         } else {
             const float* p = inference ? infer_params() : master;
-            timed(NPMG_TIME_QUERY, s, [&] {
+            run(NPMG_TIME_QUERY, s, [&] {
                 hip_check(launch_query_fp32(cfg.d, inference ? wt_infer : wt_train, p + cfg.n_mlp, io, n, n_cu, s),
                           "query (fp32)");
             });
@@ -822,6 +862,7 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        ctx->noalloc_scratch = scratch_on_device(dev);
         hip_check(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking), "stream");
         hip_check(hipStreamCreateWithFlags(&ctx->capture, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
@@ -1013,6 +1054,39 @@ npmg_status npmg_sample_pdf(npmg_ctx* ctx, const float* in, const float* normal,
         io.out_lobe = out_lobe;
         io.raw_out = raw_out;
         ctx->query(io, n, true, S(stream));
+    });
+}
+
+npmg_status npmg_sample_pdf_noalloc(npmg_ctx* ctx, const float* in, const float* normal, const float* u,
+                                    const float* eval_dirs, uint32_t n_eval, float* out_dir, float* out_pdf,
+                                    float* out_eval_pdf, uint32_t* out_lobe, float* raw_out, uint32_t n,
+                                    uint32_t flags, npmg_stream stream) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->check_fused();
+        need(n == 0 || in, "null input");
+        need(!u || (out_dir && out_pdf), "sampling needs out_dir and out_pdf");
+        need(n_eval == 0 || (eval_dirs && out_eval_pdf), "evaluation needs eval_dirs and out_eval_pdf");
+        need(!(flags & NPMG_COSINE_AWARE) || normal, "NPMG_COSINE_AWARE needs normal");
+        need(!(flags & NPMG_COSINE_AWARE) || ctx->cfg.d.mixture != MIX_ASG,
+             "the anisotropic mixture has no cosine product");
+        const DevCfg& d = ctx->cfg.d;
+        const size_t bytes = ctx->half() && mfma_split_grid(d) ? (size_t)d.n_levels * d.F * n * 2u : 0u;
+        need(bytes <= kNoAllocScratchBytes,
+             "npmg_sample_pdf_noalloc: n x grid features x 2 bytes exceeds its 16 MiB scratch (262144 queries of "
+             "a 32-feature grid); split the batch");
+        QueryIO io{};
+        io.in = in;
+        io.normal = (flags & NPMG_COSINE_AWARE) ? normal : nullptr;
+        io.u = u;
+        io.eval_dirs = eval_dirs;
+        io.n_eval = (int)n_eval;
+        io.out_dir = out_dir;
+        io.out_pdf = out_pdf;
+        io.out_eval_pdf = out_eval_pdf;
+        io.out_lobe = out_lobe;
+        io.raw_out = raw_out;
+        ctx->query(io, n, true, S(stream), ctx->noalloc_scratch);
     });
 }
 

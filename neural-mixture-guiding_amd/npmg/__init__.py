@@ -90,6 +90,7 @@ SIGNATURES = {
     "npmg_pack_training": (_int, [_vp, ctypes.POINTER(NpmgPathRecords), ctypes.POINTER(NpmgTrainTile),
                                   ctypes.POINTER(_f32), _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "npmg_sample_pdf_ex": (_int, [_vp, ctypes.POINTER(NpmgQueue), _u32, _u32, _u32, _vp]),
+    "npmg_sample_pdf_noalloc": (_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "npmg_train_step_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _int, _vp, _vp]),
     "npmg_mixture_adjoint_selection": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _f32, _vp, _vp, _vp]),
     "npmg_selection_probability": (_int, [_vp, _vp, _vp, _u32, _vp]),
