@@ -1,5 +1,9 @@
 """Benchmark of the guiding hot path on one MI355X.
 
+Headline precision "bf16x3": BF16 MFMA on split operands (hi + lo, three
+products, FP32 accumulation) with FP16 tables, the mode whose pdfs stay within
+1e-3 of FP64; plain BF16, FP16, FP8 and FP32 are timed beside it.
+
 Workload (full-frame wavefront size): hash grid 16 levels x 2^19 entries x 2
 features, OneBlob(16) over the 6 direction/normal/roughness inputs, MLP
 3 x 64 (bias-free, ReLU), K = 8 vMF lobes.
@@ -34,6 +38,13 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # in, u, normal, 4 eval dirs; dir, pdf, 4 pdfs, lobe
 BYTES_PER_QUERY = 36 + 12 + 12 + 48 + 12 + 4 + 16 + 4
 GRID_FEATURES = 16 * 2  # levels x features of the full configuration
+PRECISION_NOTE = {
+    "fp32": "FP32 VALU, bit-exact with the FP32 oracle",
+    "bf16": "BF16 MFMA operands, FP32 accumulation",
+    "fp16": "FP16 MFMA operands, FP32 accumulation",
+    "bf16x3": "BF16 MFMA on split operands (hi = bf16(x), lo = bf16(x - hi)), three products per tile "
+              "(hi hi + hi lo + lo hi), FP32 accumulation; FP16 grid table and level-major features",
+}
 
 
 def profiled(precision, key, queries_per_launch, kernel="k_query"):
@@ -72,31 +83,81 @@ def time_region(fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps  # ms per step
 
 
-def cpu_baseline(cfg, params, q_host, budget_s=10.0):
-    """The oracle's FP32 path on the host cores, on a bounded sample of the same queries."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def _host_info():
+    """Host cores available to this process and the CPU model (nproc / lscpu)."""
+    import subprocess
+
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", avail) or avail)
+    model = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": avail, "os_cpu_count": os.cpu_count(), "omp_num_threads": share, "model": model,
+            "threads_used": max(1, min(avail, share))}
+
+
+def cpu_baseline(cfg, params, q_host, train_host):
+    """The oracle's FP32 path (the restatement of the reference's algorithm,
+    stable numerics) on the host cores, in the same run: infer + sample + pdf
+    (cosine product, 1 sample, 4 evaluation pdfs) on 65,536 queries (1 warm-up,
+    median of 5) and on the 2^21 benchmark queries (median of 3), and one train
+    step (forward, adjoint, MLP and grid backward, Adam over all parameters)
+    on the 2^18 benchmark records (1 warm-up on 2^14, median of 3). Rates per
+    query / per record."""
+    import statistics
+
     import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     o = pyoracle.Oracle()
     oc = pyoracle.make_cfg(cfg)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    n = 32768
-    done, elapsed, start = 0, 0.0, 0
-    while elapsed < budget_s:
-        if start + n > N_QUERIES:  # wrap around the benchmark queries
-            start = 0
-        sl = slice(start, start + n)
-        args = [np.ascontiguousarray(q_host[k][..., sl]) for k in ("inputs", "normal", "u", "eval_dirs")]
-        t0 = time.perf_counter()
-        o.query(oc, params, *args, n_threads=threads)
-        elapsed += time.perf_counter() - t0
-        done += n
-        start += n
-        n = min(n * 2, 1 << 19)
-    return {"value": done / elapsed, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{done} of the 2^21 benchmark queries in order, wrapping (FP32 oracle with stable numerics, "
-                      f"encode+MLP+mixture+sample+4 pdfs), {elapsed:.1f} s on {threads} threads"}
+    info = _host_info()
+    th = info["threads_used"]
+
+    def query_rate(n, runs):
+        args = [np.ascontiguousarray(q_host[k][..., :n]) for k in ("inputs", "normal", "u", "eval_dirs")]
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            o.query(oc, params, *args, n_threads=th)
+            times.append(time.perf_counter() - t0)
+        return n / statistics.median(times), times
+
+    query_rate(4096, 1)  # warm-up
+    r64, t64 = query_rate(65536, 5)
+    r21, t21 = query_rate(N_QUERIES, 3)
+
+    def train_rate(n, runs):
+        t = {k: np.ascontiguousarray(v[..., :n]) for k, v in train_host.items()}
+        times = []
+        for _ in range(runs):
+            p = params.copy()
+            m, v = np.zeros_like(p), np.zeros_like(p)
+            st = np.zeros(p.size, np.uint32)
+            t0 = time.perf_counter()
+            o.train_step(oc, p, m, v, st, t, n_threads=th)
+            times.append(time.perf_counter() - t0)
+        return n / statistics.median(times), times
+
+    train_rate(1 << 14, 1)  # warm-up
+    rt, tt = train_rate(N_TRAIN, 3)
+    return {"value": r21, "unit": "queries/s", "cores": th, "kind": "port",
+            "sample": (f"oracle FP32 (stable numerics) infer+sample+pdf on the 2^21 benchmark queries, "
+                       f"median of 3 runs ({', '.join(f'{x:.2f}' for x in t21)} s) on {th} threads"),
+            "queries_per_s_65536": r64, "queries_per_s_2e21": r21, "train_samples_per_s": rt,
+            "train_sample": (f"one FP32 train step (forward, adjoint, MLP and grid backward, Adam over "
+                             f"{params.size} parameters) on the 2^18 benchmark records, median of 3 "
+                             f"({', '.join(f'{x:.2f}' for x in tt)} s)"),
+            "host": info}
 
 
 def main():
@@ -104,7 +165,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "fp16", "bf16x3"],
+                    help="headline precision; bf16x3: BF16 MFMA on split (hi + lo) operands, three products, "
+                         "the mode that meets the 1e-3 pdf error bar")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="all", choices=["all", "query", "frame"],
                     help="query: time only the fused query kernel (for counter runs); "
@@ -298,9 +361,11 @@ def main():
                 "tuning": {"pipeline_chunks": tuned[0], "blocks_per_cu": tuned[1]}}
 
     per_prec, per_prec_train, per_prec_graph, per_kernel = {}, {}, {}, {}
+    # the anisotropic frame's 128-wide network: split operands do not fit its weights in LDS
+    fprec = "bf16" if args.precision == "bf16x3" else args.precision
     if args.only == "frame":
-        frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
-        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, args.precision, False, train_steps=100)
+        frame = {p: measure_frame(p) for p in ("fp32", fprec)}
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, fprec, False, train_steps=100)
         print(json.dumps(frame), flush=True)
         return
     precs = [args.precision] if args.precision == "fp32" else ["fp32", args.precision]
@@ -338,7 +403,10 @@ def main():
     genc_bytes = (GRID_FEATURES * 2) if half else 0  # level-major grid features read back (BF16)
     table_bytes = 0 if half else net.n_grid * 4       # FP32 path gathers the table itself
     pos_bytes = 12 if half else 0                     # MFMA path: the positions are the encode's input
-    alg_bytes = (BYTES_PER_QUERY - pos_bytes + genc_bytes) * q_per_launch + table_bytes
+    io_bytes = (BYTES_PER_QUERY - pos_bytes + genc_bytes) * q_per_launch + table_bytes  # what the kernel moves
+    # algorithmic bytes of the launch: the path's 132 B per query over the queries the
+    # launch serves, plus the grid table once (16-bit tables for the MFMA paths)
+    alg_bytes = 132 * q_per_launch + net.n_grid * (2 if half else 4)
     achieved = alg_bytes / (kq["avg_ms_per_launch"] * 1e-3) / 1e9
     flops = 2 * net.n_mlp * q_per_launch
     traffic = profiled(args.precision, "traffic_bytes", q_per_launch)  # FETCH_SIZE + WRITE_SIZE, calibrated
@@ -346,7 +414,7 @@ def main():
                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                       "kernel": ("k_query_mfma" if half else "k_query_fp32") +
                                 " (fused OneBlob encode + MLP + mixture + sample + pdf)",
-                      "algorithmic_bytes_per_launch": alg_bytes,
+                      "algorithmic_bytes_per_launch": alg_bytes, "kernel_io_bytes_per_launch": io_bytes,
                       "avg_launch_ms": kq["avg_ms_per_launch"], "launches_per_step": kq["launches_per_step"]}
     roofline = fused_roofline
     ke = per_kernel[args.precision].get("grid_encode")
@@ -355,19 +423,46 @@ def main():
         # level-major grid encode: positions (12 B) in and L F BF16 features (64 B) out per
         # query, the BF16 table (26 MB) read once
         qe = N_QUERIES / ke["launches_per_step"]
-        e_bytes = (12 + GRID_FEATURES * 2) * qe + net.n_grid * 2
+        e_io = (12 + GRID_FEATURES * 2) * qe + net.n_grid * 2  # positions in, features out, table once
+        e_bytes = 132 * qe + net.n_grid * 2
         e_ach = e_bytes / (ke["avg_ms_per_launch"] * 1e-3) / 1e9
         encode_roofline = {"bound": "hbm", "achieved": e_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": e_ach / HBM_PEAK_GBS,
                            "traffic": profiled(args.precision, "traffic_bytes", qe, "k_grid_encode"),
                            "kernel": "k_grid_encode (level-major hash-grid gathers, 16 levels x 8 corners)",
-                           "algorithmic_bytes_per_launch": e_bytes,
+                           "algorithmic_bytes_per_launch": e_bytes, "kernel_io_bytes_per_launch": e_io,
                            "avg_launch_ms": ke["avg_ms_per_launch"], "launches_per_step": ke["launches_per_step"],
                            "limiter": "L1 cache-line lookups of random gathers (about 5 per query and level), "
                                       "not HBM bytes",
                            "l1_accesses_per_launch": profiled(args.precision, "l1_accesses", qe, "k_grid_encode")}
         if ke["avg_ms_per_launch"] * ke["launches_per_step"] > kq["avg_ms_per_launch"] * kq["launches_per_step"]:
             roofline = encode_roofline  # the dominant kernel of the step
+    # Whole-step figures on the per-unit algorithmic work of the step: infer +
+    # sample + pdf 132 B of query I/O per query plus the grid table once (16-bit
+    # or FP32); train step 52 B of record I/O per sample, Adam + EMA at 42 B per
+    # parameter, gradient zero and scatter 4 B per parameter, the 16-bit table
+    # read once; MLP flops 2 n_mlp per query (forward) and 6 n_mlp per sample
+    # (forward, input and weight gradients).
+    def step_figures(ms, half_table):
+        b = 132 * N_QUERIES + net.n_grid * (2 if half_table else 4)
+        return {"bound": "hbm", "bytes_per_step": b, "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "mlp_flops_per_step": 2 * net.n_mlp * N_QUERIES,
+                "mlp_tflops": 2 * net.n_mlp * N_QUERIES / (ms * 1e-3) / 1e12}
+
+    step_roofline = {p: step_figures(per_prec[p], p != "fp32") for p in per_prec if p in ("fp32", args.precision)}
+    n_all = net.n_mlp + net.n_grid
+    t_bytes = 52 * N_TRAIN + 42 * n_all + 4 * n_all + 2 * net.n_grid
+    t_flops = 6 * net.n_mlp * N_TRAIN
+
+    def train_figures(ms, mfma):
+        return {"bound": "hbm", "bytes_per_step": t_bytes, "achieved": t_bytes / (ms * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": t_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "mlp_flops_per_step": t_flops, "mlp_tflops": t_flops / (ms * 1e-3) / 1e12,
+                "mlp_peak_tflops": MFMA_BF16_PEAK_TFLOPS if mfma else FP32_PEAK_TFLOPS,
+                "ms_per_step": ms}
+
+    train_roofline = {p: train_figures(v, p != "fp32") for p, v in per_prec_train.items()}
     result = {
         "metric": METRIC,
         "value": qps,
@@ -379,11 +474,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": "bf16" if args.precision == "bf16x3" else args.precision,
         "data": "synthetic (pcg32-keyed hit records, analytic radiance field)",
         "config": {"workload": "infer+sample+pdf of 2^21 queries, hash grid 16x2^19x2 + OneBlob(16)x6, "
                                "MLP 3x64, 8 vMF lobes, cosine product, 4 eval directions, fast mixture numerics",
-                   "queries_per_step": N_QUERIES, "train_batch": N_TRAIN, "precision": args.precision},
+                   "queries_per_step": N_QUERIES, "train_batch": N_TRAIN, "precision": args.precision,
+                   "operands": PRECISION_NOTE[args.precision]},
+        "step_roofline": step_roofline,
+        "train_roofline": train_roofline,
         "train_samples_per_s": sps,
         "train_ms_per_step": ms_t,
         "roofline": roofline,
@@ -414,12 +512,12 @@ def main():
         result["pdf_rel_diff_vs_fp32_same_params"] = {
             p: pdf_agreement(full_cfg, p, True) for p in ("bf16", "fp16", "bf16x3", "fp8")}
         result["fp8_inference_speedup_vs_fp32"] = per_prec["fp32"] / per_prec["fp8"]
-        frame = {p: measure_frame(p) for p in ("fp32", args.precision)}
+        frame = {p: measure_frame(p) for p in ("fp32", fprec)}
         frame["workload"] = ("one frame of 8 bounce queues of floor(2,073,600 * 0.68^b) queries, one persistent "
                              "launch; anisotropic mixture 16 lobes x 6, MLP 3x128, hash grid 16x2^19x2, "
                              "sample + 4 eval pdfs; train step 2^18 samples as one graph")
-        frame[f"{args.precision}_speedup_vs_fp32"] = frame["fp32"]["ms_per_frame"] / frame[args.precision]["ms_per_frame"]
-        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, args.precision, False, train_steps=100)
+        frame[f"{fprec}_speedup_vs_fp32"] = frame["fp32"]["ms_per_frame"] / frame[fprec]["ms_per_frame"]
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, fprec, False, train_steps=100)
         result["anisotropic_frame"] = frame
     if "fp32" in per_prec and half:
         result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]
@@ -427,8 +525,11 @@ def main():
     if not args.no_cpu:
         params = net.get_params(1).cpu().numpy()
         q_host = {k: v.cpu().numpy() for k, v in q.items()}
-        result["cpu_baseline"] = cpu_baseline(net.config, params, q_host)
-        log(f"cpu baseline: {result['cpu_baseline']['value']:.0f} queries/s")
+        t_host = {k: v.cpu().numpy() for k, v in t.items()}
+        result["cpu_baseline"] = cpu_baseline(net.config, params, q_host, t_host)
+        cb = result["cpu_baseline"]
+        log(f"cpu baseline ({cb['cores']} threads): {cb['queries_per_s_65536']:.0f} queries/s at 65536, "
+            f"{cb['value']:.0f} at 2^21, train {cb['train_samples_per_s']:.0f} samples/s")
     net.close()
     print(json.dumps(result), flush=True)
 

@@ -1157,6 +1157,125 @@ void orc_adam_step(size_t n, size_t n_matrix, float loss_scale, float lr, float 
     }
 }
 
+// One FP32 training step on n records with n_threads host threads, the CPU
+// baseline of the train step (integrator.cpp:679-699): forward (encode, MLP
+// keeping the activations), KL loss and adjoint (train.cu:69-124), MLP
+// backward with per-thread weight-gradient partials summed in thread order,
+// grid backward with levels spread over the threads (disjoint table ranges,
+// so no two threads write one entry), then Adam (adam.h:47-119). Parameters,
+// moments and step counts are updated in place; returns the mean loss.
+float orc_train_step_f(const orc_cfg* c, float* params, float* m, float* v, uint32_t* steps, const float* in,
+                       const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf, int n, float loss_scale,
+                       float lr, float beta1, float beta2, float eps, float l2_reg, int n_threads) {
+    const EncLayout L = enc_layout(c);
+    const GridLayout g = grid_layout(c);
+    const NetLayout N = net_layout(c);
+    const int H = N.n_layers - 1, W = c->width, n_out = N.out_w[H];
+    const size_t n_mlp = N.n_params, n_grid = (size_t)g.offsets[c->n_levels] * c->n_features;
+    const float* grid = params + n_mlp;
+    const int T = std::max(1, n_threads);
+    std::vector<float> enc((size_t)L.padded * n), acts((size_t)H * W * n), dout((size_t)n_out * n), loss(n);
+    const float ls = loss_scale / (float)n;
+    const float clamp_hi = c->grad_clamp > 0 ? (float)c->grad_clamp * (loss_scale / (float)c->clamp_scale) : 0.0f;
+    // forward + adjoint, samples spread over the threads
+    parallel_for(n, T, [&](int q) {
+        float in_q[16];
+        for (int d = 0; d < c->n_input_dims; ++d) in_q[d] = in[(size_t)d * n + q];
+        std::vector<float> e(L.padded), raw(n_out), a((size_t)H * W), gq(n_out);
+        encode_one<float>(c, L, g, grid, in_q, e.data());
+        mlp_one<float>(N, params, e.data(), raw.data(), a.data());
+        for (int i = 0; i < L.padded; ++i) enc[(size_t)i * n + q] = e[i];
+        for (size_t i = 0; i < a.size(); ++i) acts[i * n + q] = a[i];
+        const float rgb[3] = {tgt_rgb[q], tgt_rgb[(size_t)n + q], tgt_rgb[2 * (size_t)n + q]};
+        adjoint_one<float>(c, raw.data(), tgt_dir[q], tgt_dir[(size_t)n + q], rgb, tgt_pdf[q], 0.0f, ls, clamp_hi,
+                           gq.data(), &loss[q]);
+        for (int j = 0; j < n_out; ++j) dout[(size_t)j * n + q] = gq[j];
+    });
+    // MLP backward: thread t owns a contiguous range of samples and a partial dW
+    std::vector<float> grad(n_mlp + n_grid, 0.0f), denc((size_t)L.padded * n);
+    std::vector<std::vector<float>> part(T, std::vector<float>(n_mlp, 0.0f));
+    {
+        std::vector<std::thread> th;
+        const int chunk = (n + T - 1) / T;
+        for (int t = 0; t < T; ++t) {
+            const int b = t * chunk, e = std::min(n, b + chunk);
+            if (b >= e) break;
+            th.emplace_back([&, t, b, e] {
+                float* dw = part[t].data();
+                std::vector<float> gg, gprev;
+                for (int q = b; q < e; ++q) {
+                    gg.assign(n_out, 0.0f);
+                    for (int o = 0; o < n_out; ++o) gg[o] = dout[(size_t)o * n + q];
+                    for (int l = H; l >= 0; --l) {
+                        const int iw = N.in_w[l], ow = N.out_w[l];
+                        auto xin = [&](int i) -> float {
+                            return l == 0 ? enc[(size_t)i * n + q] : acts[((size_t)(l - 1) * W + i) * n + q];
+                        };
+                        for (int o = 0; o < ow; ++o)
+                            for (int i = 0; i < iw; ++i) dw[N.off[l] + (size_t)o * iw + i] += gg[o] * xin(i);
+                        gprev.assign(iw, 0.0f);
+                        for (int i = 0; i < iw; ++i) {
+                            float acc = 0;
+                            for (int o = 0; o < ow; ++o) acc = fm::fma_(params[N.off[l] + (size_t)o * iw + i], gg[o], acc);
+                            gprev[i] = (l > 0 && !(xin(i) > 0.0f)) ? 0.0f : acc;
+                        }
+                        gg.swap(gprev);
+                    }
+                    for (int i = 0; i < N.in_w[0]; ++i) denc[(size_t)i * n + q] = gg[i];
+                }
+            });
+        }
+        for (auto& x : th) x.join();
+    }
+    for (int t = 0; t < T; ++t)
+        for (size_t i = 0; i < n_mlp; ++i) grad[i] += part[t][i];
+    // grid backward: level l on thread l mod T
+    int ge = -1;
+    for (int e = 0; e < c->n_enc; ++e) if (c->enc_type[e] == ORC_ENC_GRID) ge = e;
+    if (ge >= 0) {
+        std::vector<std::thread> th;
+        const int F = c->n_features;
+        for (int t = 0; t < std::min(T, c->n_levels); ++t) {
+            th.emplace_back([&, t] {
+                uint32_t idx[8];
+                float w[8];
+                for (int l = t; l < c->n_levels; l += T) {
+                    float* gl = grad.data() + n_mlp + (size_t)g.offsets[l] * F;
+                    for (int q = 0; q < n; ++q) {
+                        float pos[3];
+                        for (int d = 0; d < 3; ++d) pos[d] = in[(size_t)(c->enc_begin[ge] + d) * n + q];
+                        const int nc = level_corners<float>(c, g, l, pos, idx, w);
+                        for (int k = 0; k < nc; ++k)
+                            for (int f = 0; f < F; ++f)
+                                gl[(size_t)idx[k] * F + f] += w[k] * denc[(size_t)(L.offset[ge] + l * F + f) * n + q];
+                    }
+                }
+            });
+        }
+        for (auto& x : th) x.join();
+    }
+    // Adam over the parameters, in T contiguous ranges
+    const size_t np = n_mlp + n_grid;
+    {
+        std::vector<std::thread> th;
+        const size_t chunk = (np + T - 1) / T;
+        for (int t = 0; t < T; ++t) {
+            const size_t b = t * chunk, e = std::min(np, b + chunk);
+            if (b >= e) break;
+            th.emplace_back([&, b, e] {
+                // the matrix/grid boundary relative to this range
+                const size_t nm = n_mlp > b ? std::min(n_mlp, e) - b : 0;
+                orc_adam_step(e - b, nm, loss_scale, lr, beta1, beta2, eps, l2_reg, params + b, grad.data() + b,
+                              m + b, v + b, steps + b);
+            });
+        }
+        for (auto& x : th) x.join();
+    }
+    double sum = 0;
+    for (int q = 0; q < n; ++q) sum += loss[q];
+    return (float)(sum / n);
+}
+
 // tcnn optimizers/ema.h:62-76 and :102-136.
 void orc_ema_step(size_t n, float decay, uint32_t step, const float* weights, float* ema) {
     const float deb_old = 1 - (float)std::pow(decay, step - 1);

@@ -116,6 +116,12 @@ void orc_adam_step(size_t n, size_t n_matrix, float loss_scale, float lr, float 
                    uint32_t* steps);
 /* EMA as tcnn's Ema optimizer; step is the nested optimizer's step count after the update. */
 void orc_ema_step(size_t n, float decay, uint32_t step, const float* weights, float* ema);
+/* One FP32 training step (forward, adjoint, MLP and grid backward, Adam) on n
+ * records with n_threads host threads (the CPU baseline); updates params, m,
+ * v, steps in place and returns the mean loss. */
+float orc_train_step_f(const orc_cfg* c, float* params, float* m, float* v, uint32_t* steps, const float* in,
+                       const float* tgt_dir, const float* tgt_rgb, const float* tgt_pdf, int n, float loss_scale,
+                       float lr, float beta1, float beta2, float eps, float l2_reg, int n_threads);
 
 /* Parameter initialisation of tcnn's Trainer: std::seed_seq{seed} -> pcg32,
  * Xavier-uniform matrices then the grid table U(-1e-4, 1e-4); repeated `passes` times. */

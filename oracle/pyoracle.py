@@ -118,6 +118,8 @@ class Oracle:
         L.orc_grid_layout.restype = ctypes.c_size_t
         L.orc_adam_step.argtypes = [ctypes.c_size_t, ctypes.c_size_t, _f, _f, _f, _f, _f, _f, _vp, _vp, _vp, _vp, _vp]
         L.orc_ema_step.argtypes = [ctypes.c_size_t, _f, ctypes.c_uint32, _vp, _vp]
+        L.orc_train_step_f.argtypes = [_vp] * 9 + [ctypes.c_int, _f, _f, _f, _f, _f, _f, ctypes.c_int]
+        L.orc_train_step_f.restype = _f
         L.orc_adjoint_f.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _f, _vp, _vp]
         L.orc_adjoint_d.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp]
         L.orc_synth_queries.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp, _vp]
@@ -278,6 +280,15 @@ class Oracle:
     def adam_step(self, n_matrix, loss_scale, lr, beta1, beta2, eps, l2, master, grad, m, v, steps):
         self.lib.orc_adam_step(master.size, n_matrix, loss_scale, lr, beta1, beta2, eps, l2, _p(master), _p(grad),
                                _p(m), _p(v), _p(steps))
+
+    def train_step(self, cfg, params, m, v, steps, t, loss_scale=128.0, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-15,
+                   l2=1e-6, n_threads=8):
+        """One FP32 train step (the CPU baseline); params, m, v, steps updated in place."""
+        k = _Keep()
+        n = t["inputs"].shape[1]
+        return self.lib.orc_train_step_f(ctypes.byref(cfg), _p(params), _p(m), _p(v), _p(steps),
+                                         k(t["inputs"]), k(t["tgt_dir"]), k(t["tgt_rgb"]), k(t["tgt_pdf"]), n,
+                                         loss_scale, lr, beta1, beta2, eps, l2, n_threads)
 
     def ema_step(self, decay, step, weights, ema):
         self.lib.orc_ema_step(weights.size, decay, step, _p(weights), _p(ema))
