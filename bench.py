@@ -169,9 +169,9 @@ def main():
                     help="headline precision; bf16x3: BF16 MFMA on split (hi + lo) operands, three products, "
                          "the mode that meets the 1e-3 pdf error bar")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="all", choices=["all", "query", "frame"],
+    ap.add_argument("--only", default="all", choices=["all", "query", "frame", "train"],
                     help="query: time only the fused query kernel (for counter runs); "
-                         "frame: only the anisotropic 8-bounce frame")
+                         "frame: only the anisotropic 8-bounce frame; train: only the train step of --precision")
     ap.add_argument("--tuning", default=None, metavar="CHUNKS,BLOCKS_PER_CU",
                     help="use this query launch shape instead of autotuning (counter runs: one launch "
                          "of each kernel per step)")
@@ -367,6 +367,14 @@ def main():
         frame = {p: measure_frame(p) for p in ("fp32", fprec)}
         frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, fprec, False, train_steps=100)
         print(json.dumps(frame), flush=True)
+        return
+    if args.only == "train":
+        nt = npmg.GuidingNetwork(full_cfg(args.precision), seed=7272)
+        ms_e = measure_train(nt, args.precision)
+        ms_g = measure_train(nt, args.precision, graph=True)
+        nt.close()
+        print(json.dumps({"precision": args.precision, "train_ms_per_step": ms_e, "train_graph_ms_per_step": ms_g,
+                          "train_samples_per_s": N_TRAIN / (ms_e * 1e-3)}), flush=True)
         return
     precs = [args.precision] if args.precision == "fp32" else ["fp32", args.precision]
     if args.precision != "fp32" and args.only != "query":

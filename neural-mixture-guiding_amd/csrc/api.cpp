@@ -250,6 +250,7 @@ struct npmg_ctx {
     uint32_t t_count[kTags] = {};
     hipEvent_t ev_fork = nullptr, ev_chunk[kPipe] = {};
     hipEvent_t ev_bwd = nullptr, ev_wgrad = nullptr;  // fork/join of the weight gradients (train_backward)
+    hipEvent_t ev_prep0 = nullptr, ev_prep = nullptr;  // fork/join of the grid-gradient binning (train_forward)
     // MFMA training: transposed weight fragments and the operand-type buffers
     void* bfrag_train = nullptr;
     TrainMfmaWS mws{};
@@ -276,6 +277,8 @@ struct npmg_ctx {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_bwd) (void)hipEventDestroy(ev_bwd);
         if (ev_wgrad) (void)hipEventDestroy(ev_wgrad);
+        if (ev_prep0) (void)hipEventDestroy(ev_prep0);
+        if (ev_prep) (void)hipEventDestroy(ev_prep);
         for (hipEvent_t e : ev_chunk)
             if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)master, (void*)ema, (void*)m, (void*)v, (void*)steps, (void*)grad, (void*)wt_train,
@@ -678,6 +681,16 @@ struct npmg_ctx {
     void train_forward(const float* in, const float* td, const float* tr, const float* tp, const float* tb, uint32_t n,
                        LossScale ls, hipStream_t s) {
         ensure_ws(n);
+        // The grid-gradient binning needs only the positions: it runs on aux
+        // beside the forward pass, and train_backward joins it before the scatter.
+        if (cfg.n_grid) {
+            hip_check(hipEventRecord(ev_prep0, s), "event");
+            hip_check(hipStreamWaitEvent(aux, ev_prep0, 0), "stream wait");
+            hip_check(launch_grid_scatter_prepare(cfg.d, in, n, cfg.deterministic_grid, scatter_scratch, scatter_cap,
+                                                  aux),
+                      "grid scatter (binning)");
+            hip_check(hipEventRecord(ev_prep, aux), "event");
+        }
         if (half()) {
             hip_check(launch_train_forward_mfma(cfg.d, cfg.precision, frag_train, htab_train, genc, in, mws, n, n_cu, s),
                       "train forward (mfma)");
@@ -725,9 +738,9 @@ struct npmg_ctx {
         if (fork) {
             hip_check(hipEventRecord(ev_wgrad, aux), "event");
             float* gg = dparams + cfg.n_mlp;
-            hip_check(launch_fill(gg, 0.0f, cfg.n_grid, s), "fill");
-            hip_check(launch_grid_scatter(cfg.d, in, ws.dgrid, gg, n, cfg.deterministic_grid, scatter_scratch,
-                                          scatter_cap, s),
+            hip_check(hipStreamWaitEvent(s, ev_prep, 0), "stream wait");
+            hip_check(launch_grid_scatter_finish(cfg.d, in, ws.dgrid, gg, n, cfg.deterministic_grid, scatter_scratch,
+                                                 scatter_cap, of, s),
                       "grid scatter");
             hip_check(hipStreamWaitEvent(s, ev_wgrad, 0), "stream wait");
         }
@@ -868,6 +881,8 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         hip_check(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
         hip_check(hipEventCreateWithFlags(&ctx->ev_bwd, hipEventDisableTiming), "event");
         hip_check(hipEventCreateWithFlags(&ctx->ev_wgrad, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&ctx->ev_prep0, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&ctx->ev_prep, hipEventDisableTiming), "event");
         for (hipEvent_t& e : ctx->ev_chunk) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         const Config& c = ctx->cfg;
         ctx->master = dalloc<float>(c.n_params);

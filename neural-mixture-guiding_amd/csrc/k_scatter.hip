@@ -11,7 +11,10 @@
 //               are cut into chunks of at most kChunk records;
 //   3. place    the records (local word index, F contributions; 1 + F
 //               words) at their slots (slot order inside a (bin, block) run
-//               is arbitrary);
+//               is arbitrary), and max |dL/dy| of each level;
+// Steps 1-2 read only the sample positions, so the training step runs them
+// (launch_grid_scatter_prepare) beside the forward pass, and only 3-4 wait
+// for dL/dy (launch_grid_scatter_finish).
 //   4. reduce   one workgroup per chunk accumulates its records into the
 //               tile in LDS, then writes the tile: plain stores when the bin
 //               has one chunk (the usual case at fine levels), atomics for
@@ -93,16 +96,13 @@ __device__ __forceinline__ void load_positions(const float* __restrict__ in, int
 }
 
 __global__ void __launch_bounds__(256) k_bin_count(const DevCfg c, const BinLayout b, const float* __restrict__ in,
-                                                   const float* __restrict__ dgrid, Scratch sc, uint32_t n) {
+                                                   Scratch sc, uint32_t n) {
     extern __shared__ uint32_t hist[];
-    __shared__ float wave_max[4];
     const int l = blockIdx.y;
     const uint32_t nb = b.bin_off[l + 1] - b.bin_off[l];
     for (uint32_t i = threadIdx.x; i < nb; i += 256u) hist[i] = 0u;
     __syncthreads();
     const int g = c.enc_begin[c.grid_enc];
-    const int F = c.F;
-    float mx = 0.0f;
     float px[kPerThread], py[kPerThread], pz[kPerThread];
     load_positions(in, g, n, px, py, pz);
 #pragma unroll
@@ -113,15 +113,8 @@ __global__ void __launch_bounds__(256) k_bin_count(const DevCfg c, const BinLayo
         float w[8];
         const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
         for (int k = 0; k < nc; ++k) atomicAdd(&hist[idx[k] >> b.shift], 1u);
-        for (int f = 0; f < F; ++f) mx = smax(mx, fabsf(dgrid[(size_t)(l * F + f) * n + q]));
     }
-    // max |dL/dy| of the block, then one global atomic per block (bits of
-    // non-negative floats order like the values)
-    for (int o = 32; o > 0; o >>= 1) mx = smax(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63u) == 0) wave_max[threadIdx.x >> 6] = mx;
     __syncthreads();
-    if (threadIdx.x == 0)
-        atomicMax(&sc.maxbits[l], __float_as_uint(smax(smax(wave_max[0], wave_max[1]), smax(wave_max[2], wave_max[3]))));
     for (uint32_t i = threadIdx.x; i < nb; i += 256u) sc.counts[(size_t)(b.bin_off[l] + i) * b.nblk + blockIdx.x] = hist[i];
 }
 
@@ -190,9 +183,15 @@ __global__ void __launch_bounds__(256) k_bin_chunks(const BinLayout b, uint32_t 
     if (threadIdx.x == 0) sc.chunks[-1] = make_uint4(total_ch < b.max_chunks ? total_ch : b.max_chunks, 0u, 0u, 0u);
 }
 
+// Contribution of one record: w dL/dy, or zero for a non-finite dL/dy (the
+// overflow flag then skips the step under dynamic loss scaling).
+__device__ __forceinline__ float contrib(float w, float d) { return __builtin_isfinite(d) ? w * d : 0.0f; }
+
 __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayout b, const float* __restrict__ in,
-                                                   const float* __restrict__ dgrid, Scratch sc, uint32_t n) {
+                                                   const float* __restrict__ dgrid, Scratch sc, uint32_t n,
+                                                   uint32_t* __restrict__ overflow) {
     extern __shared__ uint32_t cursor[];
+    __shared__ float wave_max[4];
     const int l = blockIdx.y;
     const uint32_t b0 = b.bin_off[l], nb = b.bin_off[l + 1] - b0;
     for (uint32_t i = threadIdx.x; i < nb; i += 256u)
@@ -201,6 +200,12 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
     const int g = c.enc_begin[c.grid_enc];
     const int F = c.F;
     const uint32_t mask = (1u << b.shift) - 1u;
+    float mx = 0.0f;
+    bool bad = false;
+    auto seen = [&](float d) {
+        if (__builtin_isfinite(d)) mx = smax(mx, fabsf(d));
+        else bad = true;
+    };
     float px[kPerThread], py[kPerThread], pz[kPerThread];
     load_positions(in, g, n, px, py, pz);
 #pragma unroll
@@ -212,6 +217,8 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
         const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
         if (F == 2) {
             const float d0 = dgrid[(size_t)(2 * l) * n + q], d1 = dgrid[(size_t)(2 * l + 1) * n + q];
+            seen(d0);
+            seen(d1);
             if (nc == 8) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) slot[k] = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
@@ -219,25 +226,36 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
                 for (int k = 0; k < 8; ++k) {
                     uint32_t* r = sc.rec + (size_t)slot[k] * 3u;
                     r[0] = (idx[k] & mask) * 2u;
-                    r[1] = __float_as_uint(w[k] * d0);
-                    r[2] = __float_as_uint(w[k] * d1);
+                    r[1] = __float_as_uint(contrib(w[k], d0));
+                    r[2] = __float_as_uint(contrib(w[k], d1));
                 }
             } else {
                 for (int k = 0; k < nc; ++k) {
                     uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * 3u;
                     r[0] = (idx[k] & mask) * 2u;
-                    r[1] = __float_as_uint(w[k] * d0);
-                    r[2] = __float_as_uint(w[k] * d1);
+                    r[1] = __float_as_uint(contrib(w[k], d0));
+                    r[2] = __float_as_uint(contrib(w[k], d1));
                 }
             }
         } else {
+            for (int f = 0; f < F; ++f) seen(dgrid[(size_t)(l * F + f) * n + q]);
             for (int k = 0; k < nc; ++k) {
                 uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * (1u + F);
                 r[0] = (idx[k] & mask) * (uint32_t)F;
-                for (int f = 0; f < F; ++f) r[1 + f] = __float_as_uint(w[k] * dgrid[(size_t)(l * F + f) * n + q]);
+                for (int f = 0; f < F; ++f)
+                    r[1 + f] = __float_as_uint(contrib(w[k], dgrid[(size_t)(l * F + f) * n + q]));
             }
         }
     }
+    // max |dL/dy| of the block, then one global atomic per block (bits of
+    // non-negative floats order like the values); the reduce derives the
+    // level's fixed-point scale from it
+    for (int o = 32; o > 0; o >>= 1) mx = smax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63u) == 0) wave_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(&sc.maxbits[l], __float_as_uint(smax(smax(wave_max[0], wave_max[1]), smax(wave_max[2], wave_max[3]))));
+    if (bad && overflow) atomicOr(overflow, 1u);
 }
 
 template <bool DET>
@@ -395,26 +413,58 @@ size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic) {
     return ((size_t)c.offsets[c.n_levels] * c.F * 8u + 255u) / 256u * 256u;
 }
 
-hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
-                               bool deterministic, void* scratch, size_t scratch_bytes, hipStream_t s) {
-    if (n == 0 || c.grid_enc < 0) return hipSuccess;
+namespace {
+// Layout and scratch of one call, or an error for a batch beyond the limits.
+hipError_t scatter_setup(const DevCfg& c, uint32_t n, bool deterministic, void* scratch, size_t scratch_bytes,
+                         BinLayout* b, uint32_t* n_bins, Scratch* sc) {
     if ((size_t)n * c.n_levels * 8u >= 0xffffffffull || c.F > 8) return hipErrorInvalidValue;
-    uint32_t n_bins;
-    const BinLayout b = make_layout(c, n, &n_bins);
+    *b = make_layout(c, n, n_bins);
     for (int l = 0; l < c.n_levels; ++l)
-        if (b.bin_off[l + 1] - b.bin_off[l] > kMaxBinsPerLevel) return hipErrorInvalidValue;
+        if (b->bin_off[l + 1] - b->bin_off[l] > kMaxBinsPerLevel) return hipErrorInvalidValue;
     size_t need = 0;
-    const Scratch sc = carve(c, b, n_bins, n, deterministic, scratch, &need);
+    *sc = carve(c, *b, *n_bins, n, deterministic, scratch, &need);
     if (!scratch || need > scratch_bytes) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+size_t hist_bytes(const DevCfg& c, const BinLayout& b) {
     uint32_t bmax = 0;
     for (int l = 0; l < c.n_levels; ++l) bmax = std::max(bmax, b.bin_off[l + 1] - b.bin_off[l]);
-    const size_t hist_lds = (size_t)bmax * 4u;
+    return (size_t)bmax * 4u;
+}
+}  // namespace
+
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
+
+hipError_t launch_grid_scatter_prepare(const DevCfg& c, const float* in, uint32_t n, bool deterministic,
+                                       void* scratch, size_t scratch_bytes, hipStream_t s) {
+    if (n == 0 || c.grid_enc < 0) return hipSuccess;
+    BinLayout b;
+    uint32_t n_bins;
+    Scratch sc;
+    hipError_t e = scatter_setup(c, n, deterministic, scratch, scratch_bytes, &b, &n_bins, &sc);
+    if (e != hipSuccess) return e;
     const dim3 grid(b.nblk, c.n_levels);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, sc.maxbits, 32u);
-    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), hist_bytes(c, b), s, c, b, in, sc, n);
     hipLaunchKernelGGL(k_bin_scan, dim3(n_bins), dim3(256), 0, s, b, sc);
     hipLaunchKernelGGL(k_bin_chunks, dim3(1), dim3(256), 0, s, b, n_bins, sc);
-    hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_lds, s, c, b, in, dgrid, sc, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_scatter_finish(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
+                                      bool deterministic, void* scratch, size_t scratch_bytes, uint32_t* overflow,
+                                      hipStream_t s) {
+    if (c.grid_enc < 0) return hipSuccess;
+    hipError_t e = launch_fill(grad, 0.0f, (size_t)c.offsets[c.n_levels] * c.F, s);
+    if (e != hipSuccess || n == 0) return e;
+    BinLayout b;
+    uint32_t n_bins;
+    Scratch sc;
+    e = scatter_setup(c, n, deterministic, scratch, scratch_bytes, &b, &n_bins, &sc);
+    if (e != hipSuccess) return e;
+    const dim3 grid(b.nblk, c.n_levels);
+    hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_bytes(c, b), s, c, b, in, dgrid, sc, n, overflow);
     if (deterministic) {
         hipLaunchKernelGGL(k_bin_reduce<true>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
         hipLaunchKernelGGL(k_bin_finish, dim3(n_bins), dim3(256), 0, s, c, b, sc, grad, n);
@@ -422,6 +472,14 @@ hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dg
         hipLaunchKernelGGL(k_bin_reduce<false>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grad, uint32_t n,
+                               bool deterministic, void* scratch, size_t scratch_bytes, uint32_t* overflow,
+                               hipStream_t s) {
+    hipError_t e = launch_grid_scatter_prepare(c, in, n, deterministic, scratch, scratch_bytes, s);
+    if (e != hipSuccess) return e;
+    return launch_grid_scatter_finish(c, in, dgrid, grad, n, deterministic, scratch, scratch_bytes, overflow, s);
 }
 
 }  // namespace npmg

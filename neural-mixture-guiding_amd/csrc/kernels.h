@@ -112,10 +112,14 @@ hipError_t launch_train_forward_fp32(const DevCfg& c, const float* wt, const flo
 hipError_t launch_train_backward_fp32(const DevCfg& c, const float* w, const TrainWS& ws, float* denc, float* dgrid,
                                       uint32_t n, int n_cu, hipStream_t s);
 // Grid-gradient scatter (tiny-cuda-nn kernel_grid_backward, grid.h:386-509),
-// binned by table tile and reduced in LDS; grid_grad must be zero on entry.
+// binned by table tile and reduced in LDS; every word of grid_grad is written.
 // Deterministic mode sums exactly in int64 fixed point. Scratch: at least
 // grid_scatter_bytes; its first grid_scatter_clear_bytes must be zero before
-// the first call (the scatter leaves them zero again).
+// the first call (the scatter leaves them zero again). A non-finite dL/dy
+// contributes zero and raises *overflow when it is given. The binning needs
+// only the positions: launch_grid_scatter_prepare (in, n) may run before
+// dgrid exists, then launch_grid_scatter_finish with the same in, n and
+// scratch; launch_grid_scatter is both.
 size_t grid_scatter_bytes(const DevCfg& c, uint32_t n, bool deterministic);
 // Wavefront packers (train.cu:14-67, 174-191). Scene box as min xyz, max xyz.
 struct SceneBox {
@@ -161,7 +165,13 @@ hipError_t launch_fp8_amax(const DevCfg& c, const float* enc, const float* acts,
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 size_t grid_scatter_clear_bytes(const DevCfg& c, bool deterministic);
 hipError_t launch_grid_scatter(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad, uint32_t n,
-                               bool deterministic, void* scratch, size_t scratch_bytes, hipStream_t s);
+                               bool deterministic, void* scratch, size_t scratch_bytes, uint32_t* overflow,
+                               hipStream_t s);
+hipError_t launch_grid_scatter_prepare(const DevCfg& c, const float* in, uint32_t n, bool deterministic,
+                                       void* scratch, size_t scratch_bytes, hipStream_t s);
+hipError_t launch_grid_scatter_finish(const DevCfg& c, const float* in, const float* dgrid, float* grid_grad,
+                                      uint32_t n, bool deterministic, void* scratch, size_t scratch_bytes,
+                                      uint32_t* overflow, hipStream_t s);
 // Weight gradients dW_l = sum_q g_l x_l^T, deterministic split-K reduction.
 // Non-finite sums raise *overflow when it is given.
 hipError_t launch_wgrad_fp32(const DevCfg& c, const TrainWS& ws, float* partials, size_t partials_cap, float* dw,

@@ -279,7 +279,10 @@ def test_mfma_training_converges(gpu, prec):
             ls.append(net.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"]).item())
         losses[p] = ls
         scale, step = net.train_state()
-        assert step == 60
+        if p == "fp32":
+            assert step == 60
+        else:  # dynamic scaling from 2^15 may skip an overflowing FP16 step (halving the scale)
+            assert step >= 58 and scale == 32768.0 / 2 ** (60 - step), (step, scale)
         net.close()
     first = {k: v[0] for k, v in losses.items()}
     last = {k: float(np.mean(v[-5:])) for k, v in losses.items()}
