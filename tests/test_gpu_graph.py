@@ -12,25 +12,40 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_graph_train_step_matches_eager(gpu, prec):
+def _cfg(prec, kind):
+    import npmg
+
+    if kind == "asg-full":  # the per-frame train step of the anisotropic frame: 16 lobes x 6, MLP 3 x 128
+        cfg = npmg.benchmark_config("full", precision=prec, width=128, hidden=3)
+        cfg["mixture"] = {"otype": "ASG", "n_lobes": 16, "numerics": "fast"}
+    else:
+        cfg = npmg.benchmark_config("small", precision=prec)
+    cfg["grid_gradient"] = "deterministic"
+    cfg["optimizer"] = {"otype": "Ema", "decay": 0.95, "nested": cfg["optimizer"]}
+    return cfg
+
+
+@pytest.mark.parametrize("prec,kind,n", [("fp32", "vmf", 4096), ("bf16", "vmf", 4096), ("bf16x3", "vmf", 4096),
+                                         ("bf16", "asg-full", 1 << 18)])
+def test_graph_train_step_matches_eager(gpu, prec, kind, n):
     import torch
     import npmg
 
-    cfg = npmg.benchmark_config("small", precision=prec)
-    cfg["grid_gradient"] = "deterministic"
-    cfg["optimizer"] = {"otype": "Ema", "decay": 0.95, "nested": cfg["optimizer"]}
+    cfg = _cfg(prec, kind)
     eager = npmg.GuidingNetwork(cfg, seed=3)
     graph = npmg.GuidingNetwork(cfg, seed=3)
     q = npmg.synth_queries(1, 4096)
+    cosine = kind != "asg-full"
     la = torch.empty(1, device="cuda")
     lb = torch.empty(1, device="cuda")
-    for n, seed in ((4096, 9), (4096, 9), (4096, 9), (3000, 11), (3000, 11)):
-        t = npmg.synth_train(seed, n)
+    m = n - 1096  # a second batch size: the graph is captured again
+    for nn, seed in ((n, 9), (n, 9), (n, 9), (m, 11), (m, 11)):
+        t = npmg.synth_train(seed, nn)
         for _ in range(2):
             eager.train_step(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], la)
             graph.train_step_graph(t["inputs"], t["tgt_dir"], t["tgt_rgb"], t["tgt_pdf"], lb)
-            graph.sample_pdf(q["inputs"], q["normal"], q["u"], q["eval_dirs"])  # eager work between replays
+            graph.sample_pdf(q["inputs"], q["normal"] if cosine else None, q["u"], q["eval_dirs"],
+                             cosine_aware=cosine)  # eager work between replays
             torch.cuda.synchronize()
             assert la.item() == lb.item()
             assert torch.equal(eager.get_params(0), graph.get_params(0))

@@ -14,16 +14,19 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("fp32", "vMF"), ("bf16", "vMF"), ("fp16", "vMF"), ("fp32", "ASG"), ("bf16", "ASG")]
+CASES = [("fp32", "vMF"), ("bf16", "vMF"), ("fp16", "vMF"), ("bf16x3", "vMF"), ("fp32", "ASG"), ("bf16", "ASG")]
+FRAME_PATHS = 2073600  # one 1920 x 1080 bounce
 
 
-def _net(prec, mix, size="small"):
+def _net(prec, mix, size="small", numerics=None):
     import torch
     import npmg
 
     if mix == "ASG":
         c = npmg.benchmark_config(size, precision=prec, width=128, hidden=3)
         c["mixture"] = {"otype": "ASG", "n_lobes": 16}
+        if numerics:
+            c["mixture"]["numerics"] = numerics
     else:
         c = npmg.benchmark_config(size, precision=prec)
     net = npmg.GuidingNetwork(c, seed=5)
@@ -99,6 +102,32 @@ def test_queues_full_frame_bf16(gpu):
     net.close()
 
 
+def test_queues_full_size_anisotropic_frame(gpu):
+    """The anisotropic frame at its real size: 8 bounce queues of
+    floor(2,073,600 x 0.68^b) queries (6,183,755 in all, capacity 2,073,600
+    each) in one persistent launch of the 128-wide BF16 network with 16
+    anisotropic lobes, bit for bit equal to one npmg_sample_pdf per queue."""
+    import torch
+
+    net = _net("bf16", "ASG", "full", numerics="fast")
+    counts = [int(FRAME_PATHS * 0.68 ** b) for b in range(8)]
+    assert sum(counts) == 6183755
+    caps = [FRAME_PATHS] * 8
+    qs = _queues(counts, caps, 900, False)
+    cnt = torch.tensor(counts, dtype=torch.int32, device="cuda")
+    outs = net.sample_pdf_queues(qs, cnt, n_eval=4, cosine_aware=False)
+    torch.cuda.synchronize()
+    for q, o, n, cap in zip(qs, outs, counts, caps):
+        ref = net.sample_pdf(q["inputs"][:, :n].contiguous(), None, q["u"][:, :n].contiguous(),
+                             q["eval_dirs"][:, :, :n].contiguous(), cosine_aware=False)
+        for key in ("dir", "pdf", "lobe", "eval_pdf"):
+            assert torch.equal(o[key][..., :n].contiguous().view(torch.int32),
+                               ref[key].view(torch.int32)), (n, key)
+            assert not torch.any(o[key][..., n:] != 0), f"{key}: rows past the count were written"
+        del ref
+    net.close()
+
+
 def test_queue_limits(gpu):
     import torch
     import npmg
@@ -114,7 +143,7 @@ def test_queue_limits(gpu):
     net.close()
 
 
-@pytest.mark.parametrize("prec,mix", [("bf16", "vMF"), ("bf16", "ASG")])
+@pytest.mark.parametrize("prec,mix", [("bf16", "vMF"), ("bf16x3", "vMF"), ("bf16", "ASG")])
 def test_autotune_keeps_results(gpu, prec, mix):
     import torch
     import npmg
