@@ -186,8 +186,11 @@ npmg_status npmg_sample_pdf_queues(npmg_ctx* ctx, const npmg_queue* queues, cons
 
 /* Picks the launch shape of the query path for batches of about n queries by
  * timing the candidates on synthetic queries (pipeline chunks 1/2/4/8 of the
- * BF16/FP16 path and workgroups per CU of the fused and persistent kernels),
- * and keeps the fastest for later calls. Synchronous; allocates scratch. */
+ * MFMA paths and workgroups per CU of the fused and persistent kernels), then
+ * the work-counter claim sizes of the persistent queue launches on a
+ * synthetic frame of 8 queues decaying by 0.68 per bounce (64-query tiles per
+ * claim 1..16, grid-encode items per claim 8..64), and keeps the fastest for
+ * later calls. Synchronous; allocates scratch. */
 npmg_status npmg_autotune(npmg_ctx* ctx, uint32_t n, npmg_stream stream);
 /* The launch shape in use: pipeline chunks and workgroups per CU (0 = as
  * many as fit). */
@@ -198,6 +201,13 @@ npmg_status npmg_tuning(const npmg_ctx* ctx, uint32_t* chunks, uint32_t* blocks_
  * values are capped at what fits). Changes no result bit. No reference
  * counterpart (the reference has no launch-shape tuning). */
 npmg_status npmg_set_tuning(npmg_ctx* ctx, uint32_t chunks, uint32_t blocks_per_cu);
+/* The claim sizes of npmg_sample_pdf_queues: 64-query tiles a wave of the
+ * fused kernel takes per work-counter atomic, and (level, 256-query) items a
+ * workgroup of the grid encode takes per atomic (every wave shares one
+ * counter, whose atomics serialise). Set them without timing: tiles 1..64,
+ * items 1..256. Neither changes a result bit. No reference counterpart. */
+npmg_status npmg_queue_tuning(const npmg_ctx* ctx, uint32_t* tiles_per_claim, uint32_t* items_per_claim);
+npmg_status npmg_set_queue_tuning(npmg_ctx* ctx, uint32_t tiles_per_claim, uint32_t items_per_claim);
 
 /* FP8 inference ("inference_precision": "fp8" with "precision": "bf16"): the
  * query path's MLP runs on E4M3 weights and activations with FP32

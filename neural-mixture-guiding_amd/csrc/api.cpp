@@ -230,6 +230,9 @@ struct npmg_ctx {
     // launch shape of the query path (npmg_autotune): pipeline chunks, workgroups per CU (0: all that fit)
     uint32_t tune_chunks = 4;
     int tune_bpc = 0;
+    // persistent queue launches: 64-query tiles per work-counter claim (fused kernel)
+    // and (level, 256-query) items per claim (grid encode)
+    uint32_t tune_tiles = 4, tune_items = 16;
     uint32_t* queue_counter = nullptr;  // [2] work counters of the persistent queue launches
     void* noalloc_scratch = nullptr;    // this device's `scratch` (npmg_sample_pdf_noalloc)
     uint32_t* pack_scratch = nullptr;   // block totals of the training packer
@@ -489,10 +492,12 @@ struct npmg_ctx {
                         off += qv[i].capacity;
                     }
                     qs.counter = queue_counter;
+                    qs.batch = tune_items;
                     hip_check(launch_grid_encode_queues(cfg.d, cfg.precision, htab_infer, qs, n_cu, s),
                               "grid encode (queues)");
                 }
                 qs.counter = queue_counter + 1;
+                qs.batch = tune_tiles;
                 need(!cfg.infer_fp8 || fp8_calibrated,
                      "FP8 inference is not calibrated: call npmg_fp8_calibrate (or one plain query) first");
                 hip_check(launch_query_mfma_queues(cfg.d, cfg.precision, frag_infer, htab_infer, qs, n_cu, tune_bpc,
@@ -500,6 +505,7 @@ struct npmg_ctx {
                           "query (mfma, queues)");
             } else {
                 qs.counter = queue_counter + 1;
+                qs.batch = 1u;
                 hip_check(launch_query_fp32_queues(cfg.d, wt_infer, infer_params() + cfg.n_mlp, qs, n_cu, s),
                           "query (fp32, queues)");
             }
@@ -513,14 +519,15 @@ struct npmg_ctx {
         const bool split = mfma_split_grid(cfg.d);
         // synthetic queries: in [9][n], normal [3][n], u [3][n], eval_dirs [4][3][n]; outputs dir [3][n], pdf [n],
         // eval pdfs [4][n] (the workload of a guided bounce: sample + 4 pdfs)
-        size_t bytes = sizeof(float) * (size_t)n * (9 + 3 + 3 + 12 + 3 + 1 + 4);
+        size_t bytes = sizeof(float) * (size_t)n * (9 + 3 + 3 + 12 + 3 + 1 + 4) + 8 * sizeof(uint32_t);
         float* buf = nullptr;
         hip_check(hipMalloc((void**)&buf, bytes), "hipMalloc");
+        uint32_t* qcounts = reinterpret_cast<uint32_t*>(buf + (size_t)n * (9 + 3 + 3 + 12 + 3 + 1 + 4));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         // restored on every exit: timing is off while tuning, and the FP8 path is
         // timed without calibrating it on the synthetic queries
         const bool was_timing = timing, was_calibrated = fp8_calibrated;
-        const uint32_t old_c = tune_chunks;
+        const uint32_t old_c = tune_chunks, old_t = tune_tiles, old_i = tune_items;
         const int old_b = tune_bpc;
         try {
             float *in = buf, *nrm = in + 9 * (size_t)n, *u = nrm + 3 * (size_t)n, *ed = u + 3 * (size_t)n;
@@ -562,15 +569,54 @@ struct npmg_ctx {
                     }
                 }
             }
-            timing = was_timing;
-            fp8_calibrated = was_calibrated;
             tune_chunks = best_c;
             tune_bpc = best_b;
+            // Queue launches: the claim sizes of the work counters, timed on a
+            // frame of 8 queues whose lengths decay by 0.68 per bounce (the
+            // queues share the synthetic arrays; identical queries write
+            // identical results, so the overlap changes nothing).
+            npmg_queue qv[8];
+            uint32_t cnt_h[8];
+            double wsum = 0.0;
+            for (int b = 0; b < 8; ++b) wsum += std::pow(0.68, b);
+            for (int b = 0; b < 8; ++b) {
+                qv[b] = npmg_queue{in, cfg.d.mixture == MIX_ASG ? nullptr : nrm, u, ed, od, op, oe, nullptr, n, nullptr};
+                cnt_h[b] = (uint32_t)std::max(1.0, std::floor(n * std::pow(0.68, b) / wsum));
+            }
+            hip_check(hipMemcpyAsync(qcounts, cnt_h, sizeof(cnt_h), hipMemcpyHostToDevice, s), "upload");
+            const uint32_t flags = cfg.d.mixture == MIX_ASG ? 0u : NPMG_COSINE_AWARE;
+            best = 1e30f;
+            uint32_t best_t = tune_tiles, best_i = tune_items;
+            for (uint32_t tl : {4u, 1u, 2u, 8u, 16u}) {
+                for (uint32_t it : {16u, 8u, 32u, 64u}) {
+                    if (!split && it != 16u) continue;  // no grid encode launch to tune
+                    tune_tiles = tl;
+                    tune_items = it;
+                    query_queues(qv, qcounts, 8, 4, flags, s);  // warm-up
+                    hip_check(hipEventRecord(e0, s), "event");
+                    for (int r = 0; r < 5; ++r) query_queues(qv, qcounts, 8, 4, flags, s);
+                    hip_check(hipEventRecord(e1, s), "event");
+                    hip_check(hipEventSynchronize(e1), "event");
+                    float ms = 0.0f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+                    if (ms < best * 0.98f) {
+                        best = ms;
+                        best_t = tl;
+                        best_i = it;
+                    }
+                }
+            }
+            tune_tiles = best_t;
+            tune_items = best_i;
+            timing = was_timing;
+            fp8_calibrated = was_calibrated;
         } catch (...) {
             timing = was_timing;
             fp8_calibrated = was_calibrated;
             tune_chunks = old_c;
             tune_bpc = old_b;
+            tune_tiles = old_t;
+            tune_items = old_i;
             (void)hipStreamSynchronize(s);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
@@ -1432,6 +1478,23 @@ npmg_status npmg_set_tuning(npmg_ctx* ctx, uint32_t chunks, uint32_t blocks_per_
         need(blocks_per_cu <= 64u, "workgroups per CU must be 0..64");
         ctx->tune_chunks = chunks;
         ctx->tune_bpc = (int)blocks_per_cu;
+    });
+}
+
+npmg_status npmg_queue_tuning(const npmg_ctx* ctx, uint32_t* tiles_per_claim, uint32_t* items_per_claim) {
+    if (!ctx || !tiles_per_claim || !items_per_claim) return NPMG_EINVAL;
+    *tiles_per_claim = ctx->tune_tiles;
+    *items_per_claim = ctx->tune_items;
+    return NPMG_OK;
+}
+
+npmg_status npmg_set_queue_tuning(npmg_ctx* ctx, uint32_t tiles_per_claim, uint32_t items_per_claim) {
+    if (!ctx) return NPMG_EINVAL;
+    return guarded(ctx, [&] {
+        need(tiles_per_claim >= 1u && tiles_per_claim <= 64u, "tiles per claim must be 1..64");
+        need(items_per_claim >= 1u && items_per_claim <= 256u, "items per claim must be 1..256");
+        ctx->tune_tiles = tiles_per_claim;
+        ctx->tune_items = items_per_claim;
     });
 }
 

@@ -68,9 +68,8 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
 
 // Level-major grid encoding of several queues in one persistent launch:
 // workgroups pull (level, 256-query tile) items in level-major order, a batch
-// of kEncodeBatch consecutive items per atomic (one counter serves every
+// of qs.batch consecutive items per atomic (one counter serves every
 // workgroup; one atomic per item serialised on it and took most of the time).
-constexpr uint32_t kEncodeBatch = 16;
 
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, const T* __restrict__ table,
@@ -81,12 +80,12 @@ __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, cons
     const uint32_t per_level = first[kMaxQueues];
     const uint32_t total = per_level * (uint32_t)c.n_levels;
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(qs.counter, kEncodeBatch);
+        if (threadIdx.x == 0) item = atomicAdd(qs.counter, qs.batch);
         __syncthreads();
         const uint32_t w0 = item;
         __syncthreads();
         if (w0 >= total) break;
-        const uint32_t w1 = min(total, w0 + kEncodeBatch);
+        const uint32_t w1 = min(total, w0 + qs.batch);
         for (uint32_t w = w0; w < w1; ++w) {
             const int l = (int)(w / per_level);
             const uint32_t t = w % per_level;

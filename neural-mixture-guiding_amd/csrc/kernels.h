@@ -35,6 +35,7 @@ struct QueueSet {
     const uint32_t* counts;       // [n_queues] queries per queue (device)
     uint32_t* counter;            // work counter (device), zero before the launch
     uint32_t n_queues;
+    uint32_t batch;               // consecutive work items a workgroup or wave claims per atomic (>= 1)
 };
 
 // first[i] = tiles of `tile` queries in queues 0 .. i-1; first[n_queues] = total.

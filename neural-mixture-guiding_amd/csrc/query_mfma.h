@@ -326,9 +326,9 @@ __global__ void __launch_bounds__((64 * query_waves<W, O>())) k_query_mfma(const
 
 // Several queues in one persistent launch: each wave pulls 64-query tiles
 // from a device counter over the concatenated tiles of the queues (counts on
-// the device) until all are taken, kTileBatch consecutive tiles per atomic
-// (every wave shares the one counter, whose same-address atomics serialise).
-constexpr uint32_t kTileBatch = 4;
+// the device) until all are taken, qs.batch consecutive tiles per atomic
+// (every wave shares the one counter, whose same-address atomics serialise;
+// the autotuner picks the batch).
 
 template <class T, class O, int W, int K, int P, bool FAST>
 __global__ void __launch_bounds__((64 * query_waves<W, O>())) k_query_mfma_queues(const DevCfg c, const O* __restrict__ wfrag,
@@ -346,10 +346,10 @@ __global__ void __launch_bounds__((64 * query_waves<W, O>())) k_query_mfma_queue
     const uint32_t total = first[kMaxQueues];
     for (;;) {
         uint32_t w0 = 0;
-        if (lane == 0) w0 = atomicAdd(qs.counter, kTileBatch);
+        if (lane == 0) w0 = atomicAdd(qs.counter, qs.batch);
         w0 = __shfl(w0, 0);  // wave-uniform first tile; every wave leaves once the items run out
         if (w0 >= total) break;
-        const uint32_t w1 = min(total, w0 + kTileBatch);
+        const uint32_t w1 = min(total, w0 + qs.batch);
         for (uint32_t w = w0; w < w1; ++w) {
             int qi = 0;
             uint32_t base = 0;

@@ -103,6 +103,8 @@ SIGNATURES = {
     "npmg_fp8_scales": (_int, [_vp, _vp, _vp, _u32]),
     "npmg_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "npmg_set_tuning": (_int, [_vp, _u32, _u32]),
+    "npmg_queue_tuning": (_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "npmg_set_queue_tuning": (_int, [_vp, _u32, _u32]),
 }
 
 
@@ -548,6 +550,15 @@ class GuidingNetwork:
         c, b = _u32(), _u32()
         self._check(self._lib.npmg_tuning(self._h, ctypes.byref(c), ctypes.byref(b)))
         return c.value, b.value
+
+    def queue_tuning(self):
+        """(64-query tiles, grid-encode items) claimed per work-counter atomic by queue launches."""
+        t, i = _u32(), _u32()
+        self._check(self._lib.npmg_queue_tuning(self._h, ctypes.byref(t), ctypes.byref(i)))
+        return t.value, i.value
+
+    def set_queue_tuning(self, tiles_per_claim: int, items_per_claim: int):
+        self._check(self._lib.npmg_set_queue_tuning(self._h, int(tiles_per_claim), int(items_per_claim)))
 
     def set_tuning(self, chunks: int, blocks_per_cu: int):
         """Sets the query path's launch shape (e.g. a saved autotune result)."""

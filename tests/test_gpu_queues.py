@@ -172,4 +172,22 @@ def test_autotune_keeps_results(gpu, prec, mix):
         with pytest.raises(npmg.NpmgError):
             net.set_tuning(*bad)
     assert net.tuning() == (2, 64)
+    # the queue launches' claim sizes: the tuned ones and others give the same bits
+    tiles, items = net.queue_tuning()
+    print("queue claims", tiles, items)
+    assert 1 <= tiles <= 16 and 8 <= items <= 64
+    counts = [3000, 2040, 1387, 0, 640]
+    qs = _queues(counts, [3100] * 5, 40, cosine)
+    cnt = torch.tensor(counts, dtype=torch.int32, device="cuda")
+    ref = net.sample_pdf_queues(qs, cnt, n_eval=4, cosine_aware=cosine)
+    for shape in ((1, 1), (64, 256), (3, 7)):
+        net.set_queue_tuning(*shape)
+        assert net.queue_tuning() == shape
+        got = net.sample_pdf_queues(qs, cnt, n_eval=4, cosine_aware=cosine)
+        for a, b in zip(ref, got):
+            for k in a:
+                assert torch.equal(a[k], b[k]), (shape, k)
+    for bad in ((0, 16), (65, 16), (4, 0), (4, 257)):
+        with pytest.raises(npmg.NpmgError):
+            net.set_queue_tuning(*bad)
     net.close()
