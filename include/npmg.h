@@ -382,7 +382,9 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
 /* Evaluates the library's device elementary functions over x (and y):
  * fn 0 exp, 1 log, 2 log1p, 3 expm1, 4 sin, 5 cos, 6 acos, 7 atan2(x, y),
  * 8 sqrt, 9 logistic, 10 x / y, 11 x rounded to FP8 E4M3 and back (the
- * conversion of the FP8 inference path). For bitwise host/device comparisons. */
+ * conversion of the FP8 inference path), 12 x rounded to FP16 (round to nearest
+ * even, the rounding of the FP16 adjoint intermediates and EMA). For bitwise
+ * host/device comparisons. */
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream);
 
 /* ---- Kernel timing ---------------------------------------------------------

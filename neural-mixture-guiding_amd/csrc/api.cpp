@@ -830,6 +830,7 @@ struct npmg_ctx {
         a.l2_reg = cfg.l2_reg;
         a.ema = ema != nullptr;
         a.ema_decay = cfg.ema_decay;
+        a.ema_half = cfg.ema_full < 0 ? cfg.precision == PREC_FP16 : cfg.ema_full == 0;
         a.half_precision = half() ? cfg.precision : PREC_FP32;
         a.half_train = htab_train;
         a.half_infer = htab_infer;
@@ -1848,7 +1849,7 @@ npmg_status npmg_synth_train(uint64_t seed, uint64_t first, uint32_t n, float* i
 
 npmg_status npmg_debug_math(int fn, const float* x, const float* y, uint32_t n, float* out, npmg_stream stream) {
     if (n && !(x && out)) return NPMG_EINVAL;
-    if (fn < 0 || fn > 11) return NPMG_EINVAL;
+    if (fn < 0 || fn > 12) return NPMG_EINVAL;
     return launch_debug_math(fn, x, y, n, out, S(stream)) == hipSuccess ? NPMG_OK : NPMG_EHIP;
 }
 

@@ -210,6 +210,7 @@ Config parse_config(const std::string& text) {
         if (ieq(o->string("otype", "Adam"), "Ema")) {
             c.ema = true;
             c.ema_decay = (float)o->number("decay", 0.99);
+            if (o->contains("full_precision")) c.ema_full = o->boolean("full_precision", true) ? 1 : 0;
             if (!o->contains("nested")) bad("Ema optimizer needs 'nested'");
             o = &o->at("nested");
         } else {
@@ -234,6 +235,7 @@ Config parse_config(const std::string& text) {
         c.scale_window = (uint32_t)loss.number("scale_window", 1000);
         d.grad_clamp = (float)loss.number("grad_clamp", 150.0);
         d.clamp_scale = (float)loss.number("clamp_scale", 128.0);
+        d.half_adjoint = loss.boolean("fp16_intermediates", false) ? 1 : 0;
     } else {
         d.grad_clamp = 150.0f;
         d.clamp_scale = 128.0f;

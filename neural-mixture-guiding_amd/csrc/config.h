@@ -51,6 +51,9 @@ struct DevCfg {
     int sel;          // 1: learned selection probability in output lobe_params * K (MixedSGWithSelection)
     float cos_kappa, grad_clamp;
     float clamp_scale;  // loss scale at which grad_clamp applies (threshold grad_clamp * scale / clamp_scale)
+    // 1: the adjoint keeps the reference's FP16 intermediates ("loss": {"fp16_intermediates": true}):
+    // raw outputs read as FP16, per-lobe dp/d(param) stored in FP16, dL/draw stored in FP16
+    int half_adjoint;
 };
 
 struct Config {
@@ -63,6 +66,9 @@ struct Config {
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, l2_reg = 1e-6f;
     bool ema = true;
     float ema_decay = 0.95f;
+    // EMA recurrence in FP32 or FP16: "full_precision" true / false; absent, it
+    // runs in the parameter type like tcnn's default (FP16 for precision fp16)
+    int ema_full = -1;
     // loss scale: static (reference, parameters.h:24) or dynamic
     float loss_scale = 128.0f;
     bool dynamic_scale = false;

@@ -734,11 +734,21 @@ __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
 // distribution.h:216-231): P = s (p + eps) + (1 - s) bsdf_pdf, loss
 // -f/P log P, dL/dx = -f/P^2 dP/dx with dP/dlobe = s dp/dlobe and the
 // selection slot 0.01 (p - bsdf_pdf) s (1 - s); no NaN zeroing or clamp.
-template <int K, class Raw, class Out>
-__device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
+//
+// c.half_adjoint reproduces the reference's FP16 intermediates: the raw
+// outputs are read as the network's __half outputs (train.cu:80-81), the four
+// per-lobe dp/d(param) are stored in __half by gradients_probability
+// (distribution.h:143-181, its output is precision_t), and each dL/dx is
+// stored in __half before the NaN test and the clamp (train.cu:95-123), so a
+// value beyond the FP16 range becomes infinite and is zeroed, not clamped.
+template <int K, class Raw0, class Out>
+__device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw0&& x0, float th_t, float ph_t, float r, float g,
                                                  float b, float wipdf, float bsdf_pdf, float ls_over_n,
                                                  float clamp_hi, Out&& out) {
     const bool ref = c.numerics == NUM_REFERENCE;
+    const bool h16 = c.half_adjoint != 0;
+    auto x = [&](int j) { return h16 ? fm::round_half(x0(j)) : x0(j); };
+    auto hq = [&](float v) { return h16 ? fm::round_half(v) : v; };
     Mixture<K> m;
     build_mixture<K>(c, x, m);
     const V3 wi = sph_to_cart(th_t, ph_t);
@@ -759,14 +769,16 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
         prefix = -Li / P / P * ls_over_n;
         loss = -Li / P * fm::log(P);
         sel_s = s;
-        sel_grad = 0.01f * prefix * (p - bsdf_pdf) * (s * (1.0f - s));
+        sel_grad = hq(0.01f * prefix * hq(p - bsdf_pdf) * (s * (1.0f - s)));
     } else {
         prefix = -Li / wiPdf / guide * ls_over_n;
         loss = -Li / wiPdf * fm::log(guide);
     }
-    // dp/d(param) of a lobe, times s with a learned selection
-    auto sc = [&](float v) { return c.sel ? v * sel_s : v; };
+    // dp/d(param) of a lobe, times s with a learned selection (stored again:
+    // distribution.h:224-226)
+    auto sc = [&](float v) { return c.sel ? hq(v * sel_s) : v; };
     auto finish = [&](int j, float v) {
+        v = hq(v);
         if (!c.sel) {
             if (!fm::isfinite_(v)) v = 0.0f;
             if (clamp_hi > 0.0f) v = smin(clamp_hi, v);
@@ -781,13 +793,13 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw&& x, float
         // directions from mu and sin t (no second trigonometric evaluation).
         const float st = fm::sqrt_(smax(0.0f, mu.x * mu.x + mu.y * mu.y));
         const float wv = m.weight[s] * vmf[s];
-        const float dl = (vmf[s] - p) / m.total;
-        const float dk = wv * (dot(wi, mu) + inv_k_minus_coth(kap, ref));
+        const float dl = hq((vmf[s] - p) / m.total);
+        const float dk = hq(wv * (dot(wi, mu) + inv_k_minus_coth(kap, ref)));
         // d mu / d theta = (cos t cos p, cos t sin p, -sin t); d mu / d phi = sin t (-sin p, cos p, 0).
         float cp = 1.0f, sp = 0.0f;
         if (st > 0.0f) { cp = mu.x / st; sp = mu.y / st; }
-        const float dth = wv * kap * (mu.z * (wi.x * cp + wi.y * sp) - wi.z * st);
-        const float dph = wv * kap * (wi.y * cp - wi.x * sp) * st;
+        const float dth = hq(wv * kap * (mu.z * (wi.x * cp + wi.y * sp) - wi.z * st));
+        const float dph = hq(wv * kap * (wi.y * cp - wi.x * sp) * st);
         if (c.mapping == MAP_REFERENCE) {
             const float x0 = x(4 * s + 0), x1 = x(4 * s + 1), x2 = x(4 * s + 2), x3 = x(4 * s + 3);
             const float e0 = fm::exp(clampv(x0, -10.0f, 15.0f));

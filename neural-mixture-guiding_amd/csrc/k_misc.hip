@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(256) k_debug_math(int fn, const float* x, cons
     case 9: r = fm::logistic(a); break;
     case 10: r = a / b; break;
     case 11: r = __builtin_amdgcn_cvt_f32_fp8(__builtin_amdgcn_cvt_pk_fp8_f32(a, 0.0f, 0, false), 0); break;
+    case 12: r = fm::round_half(a); break;
     default: r = 0.0f;
     }
     out[i] = r;

@@ -23,7 +23,12 @@
 // from max |dL/dy| so that no sum can overflow (|sum| <= 8 n max < 2^62): the
 // sums are exact, so the order of the LDS atomics does not matter, and the
 // resolution is 2^-(62 - log2(8 n)) max |dL/dy| (2^-41 of the level's largest
-// contribution at n = 2^18). Integer LDS atomics are also several times faster
+// contribution at n = 2^18). Each contribution is rounded to the nearest step
+// (to_fixed), so one below half a step -- under 2^-42 of the level's largest
+// |dL/dy| (rounded up to a power of two) at n = 2^18 -- flushes to zero, in
+// every order. An FP32 atomic sum like the reference's drops a term once the
+// running sum exceeds 2^25 times it, so the fixed-point sum keeps every term
+// that FP32 accumulation would keep in practice. Integer LDS atomics are also several times faster
 // than FP32 ones on gfx950. The modes differ only for the coarse bins that
 // span several chunks: atomic mode adds each chunk's converted tile to the
 // gradient with FP32 global atomics (order-dependent rounding, like the

@@ -296,7 +296,10 @@ __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restric
         const uint32_t t = a.st->step + 1u, last = a.ema_pow_len - 1u;
         const float deb_old = 1 - a.ema_pow[t - 1u < last ? t - 1u : last];
         const float deb_new = 1.0f / (1 - a.ema_pow[t < last ? t : last]);
-        e = (ema[i] * a.ema_decay * deb_old + w * (1 - a.ema_decay)) * deb_new;
+        if (a.ema_half)
+            e = fm::round_half((ema[i] * a.ema_decay * deb_old + fm::round_half(w) * (1 - a.ema_decay)) * deb_new);
+        else
+            e = (ema[i] * a.ema_decay * deb_old + w * (1 - a.ema_decay)) * deb_new;
         ema[i] = e;
     }
     if (a.half_precision != PREC_FP32 && i >= a.n_matrix) {

@@ -186,6 +186,10 @@ struct AdamArgs {
     size_t n, n_matrix;
     float loss_scale, lr, beta1, beta2, eps, l2_reg;
     bool ema, scale_from_state, check_overflow;
+    // EMA recurrence in FP16 (tcnn's ema_step_half_precision with __half
+    // weights, optimizers/ema.h:62-76): the input weight and the stored
+    // average are rounded to FP16 each step
+    bool ema_half;
     float ema_decay;
     const ScaleState* st;
     const float* ema_pow;

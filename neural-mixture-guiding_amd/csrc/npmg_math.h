@@ -58,6 +58,24 @@ constexpr float kLn2Hi = 0.693145751953125f;          // 0x3f317200, 15 signific
 constexpr float kLn2Lo = 1.428606765330187045e-06f;   // ln2 - kLn2Hi
 constexpr float kLog2e = 1.44269504088896340736f;
 
+// x rounded to the nearest IEEE binary16 value (ties to even) and returned
+// as a float: what storing a float into a __half and reading it back gives,
+// including the binary16 subnormals (quantum 2^-24) and overflow to +-inf at
+// and above 65520.
+NPMG_HD float round_half(float x) {
+    const uint32_t u = float_to_bits(x);
+    const uint32_t sign = u & 0x80000000u, a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;                                // inf, nan
+    if (a >= 0x477ff000u) return bits_to_float(sign | 0x7f800000u);  // |x| >= 65520
+    if (a < 0x38800000u) {                                         // |x| < 2^-14: multiples of 2^-24
+        const float q = bits_to_float(a) * 16777216.0f;             // exact: |q| < 2^10
+        const float r = (q + 12582912.0f) - 12582912.0f;            // round to an integer, ties to even
+        return bits_to_float(sign | float_to_bits(r * 5.9604644775390625e-8f));
+    }
+    const uint32_t b = (a + 0xfffu + ((a >> 13) & 1u)) & 0xffffe000u;  // 10 fraction bits, ties to even
+    return bits_to_float(sign | b);
+}
+
 NPMG_HD float inf_() { return bits_to_float(0x7f800000u); }
 NPMG_HD float nan_() { return bits_to_float(0x7fc00000u); }
 NPMG_HD bool isnan_(float x) { return x != x; }

@@ -624,6 +624,17 @@ template <class T>
 void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3], float wipdf, float bsdf,
                  T ls_over_n, T clamp_hi, T* g, T* loss) {
     const bool ref = c->numerics == ORC_NUM_REFERENCE;
+    // fp16_intermediates: the reference's __half buffers (train.cu:80-81 raw
+    // outputs, distribution.h:177-178 per-lobe dp, train.cu:95-99 dL/dx).
+    const bool h16 = c->fp16_intermediates != 0;
+    auto hq = [&](T v) { return h16 ? (T)fm::round_half((float)v) : v; };
+    const int n_raw = orc_padded_output(c);
+    std::vector<T> xh;
+    if (h16) {
+        xh.resize(n_raw);
+        for (int j = 0; j < n_raw; ++j) xh[j] = hq(x[j]);
+        x = xh.data();
+    }
     Mixture<T> m;
     build_mixture(c, x, m);
     const int K = m.K;
@@ -658,10 +669,10 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
         }
         const T st = M<T>::sin(L.theta), ct = M<T>::cos(L.theta), sp = M<T>::sin(L.phi), cp = M<T>::cos(L.phi);
         const T wv = m.weight[s] * vmf[s];
-        dp[s][0] = dl;
-        dp[s][1] = wv * (wi.x * st * cp + wi.y * st * sp + wi.z * ct + ik);
-        dp[s][2] = wv * L.kappa * (ct * (wi.x * cp + wi.y * sp) - wi.z * st);
-        dp[s][3] = wv * L.kappa * (wi.y * cp - wi.x * sp) * st;
+        dp[s][0] = hq(dl);
+        dp[s][1] = hq(wv * (wi.x * st * cp + wi.y * st * sp + wi.z * ct + ik));
+        dp[s][2] = hq(wv * L.kappa * (ct * (wi.x * cp + wi.y * sp) - wi.z * st));
+        dp[s][3] = hq(wv * L.kappa * (wi.y * cp - wi.x * sp) * st);
     }
     const T guide = prob + C<T>::eps;
     const bool sel = c->learn_selection != 0;
@@ -672,7 +683,7 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
         prefix = -Li / P / P * ls_over_n;
         *loss = -Li / P * M<T>::log(P);
         for (int s = 0; s < K; ++s)
-            for (int j = 0; j < 4; ++j) dp[s][j] = dp[s][j] * s_sel;
+            for (int j = 0; j < 4; ++j) dp[s][j] = hq(dp[s][j] * s_sel);
     } else {
         prefix = -Li / wiPdf / guide * ls_over_n;
         *loss = -Li / wiPdf * M<T>::log(guide);
@@ -707,8 +718,9 @@ void adjoint_one(const orc_cfg* c, const T* x, T th_t, T ph_t, const float rgb[3
             g[4 * s + 3] = prefix * dp[s][0] * m.lobe[s].lambda;
         }
     }
+    for (int j = 0; j < 4 * K; ++j) g[j] = hq(g[j]);
     if (sel) {
-        g[4 * K] = (T)0.01 * prefix * (prob - (T)bsdf) * (s_sel * ((T)1 - s_sel));
+        g[4 * K] = hq((T)0.01 * prefix * hq(prob - (T)bsdf) * (s_sel * ((T)1 - s_sel)));
         return;
     }
     // train.cu:106-123: zero non-finite gradients, clamp from above.
@@ -1283,6 +1295,15 @@ void orc_ema_step(size_t n, float decay, uint32_t step, const float* weights, fl
     for (size_t i = 0; i < n; ++i) ema[i] = (ema[i] * decay * deb_old + weights[i] * (1 - decay)) * deb_new;
 }
 
+// ema.h:62-76: weights and weights_ema are __half; each step reads both as
+// float and stores the result as __half.
+void orc_ema_step_half(size_t n, float decay, uint32_t step, const float* weights, float* ema) {
+    const float deb_old = 1 - (float)std::pow(decay, step - 1);
+    const float deb_new = 1.0f / (1 - (float)std::pow(decay, step));
+    for (size_t i = 0; i < n; ++i)
+        ema[i] = fm::round_half((ema[i] * decay * deb_old + fm::round_half(weights[i]) * (1 - decay)) * deb_new);
+}
+
 // tcnn trainer.h:53-92, gpu_matrix.h:291-306, grid.h:1333-1336, random.h:66-97.
 void orc_init_params(const orc_cfg* c, uint32_t seed, int passes, float* params) {
     std::seed_seq seq{seed};
@@ -1385,6 +1406,7 @@ void orc_math(int fn, const float* x, const float* y, int n, float* out) {
         case 8: r = fm::sqrt_(a); break;
         case 9: r = fm::logistic(a); break;
         case 10: r = a / b; break;
+        case 12: r = fm::round_half(a); break;
         default: r = 0;
         }
         out[i] = r;

@@ -50,6 +50,8 @@ typedef struct {
     float clamp_scale;              /* loss scale at which grad_clamp applies (the reference's 128):
                                        at loss scale s the threshold is grad_clamp * s / clamp_scale */
     int learn_selection;            /* 1: output 4 K is the selection logit (MixedSGWithSelection) */
+    int fp16_intermediates;         /* 1: the adjoint rounds the raw outputs, the per-lobe dp/d(param)
+                                       and dL/draw to FP16 like the reference's __half buffers */
 } orc_cfg;
 
 /* Layout of the network: sizes of the padded encoding, MLP and grid. */
@@ -116,6 +118,9 @@ void orc_adam_step(size_t n, size_t n_matrix, float loss_scale, float lr, float 
                    uint32_t* steps);
 /* EMA as tcnn's Ema optimizer; step is the nested optimizer's step count after the update. */
 void orc_ema_step(size_t n, float decay, uint32_t step, const float* weights, float* ema);
+/* The same with tcnn's FP16 recurrence (ema_step_half_precision over __half
+ * weights): input weight and stored average rounded to FP16. */
+void orc_ema_step_half(size_t n, float decay, uint32_t step, const float* weights, float* ema);
 /* One FP32 training step (forward, adjoint, MLP and grid backward, Adam) on n
  * records with n_threads host threads (the CPU baseline); updates params, m,
  * v, steps in place and returns the mean loss. */

@@ -30,7 +30,7 @@ class OrcCfg(ctypes.Structure):
         ("width", ctypes.c_int), ("n_hidden_layers", ctypes.c_int), ("n_output_dims", ctypes.c_int),
         ("mixture", ctypes.c_int), ("n_lobes", ctypes.c_int), ("mapping", ctypes.c_int), ("numerics", ctypes.c_int),
         ("cosine_kappa", ctypes.c_float), ("grad_clamp", ctypes.c_float), ("clamp_scale", ctypes.c_float),
-        ("learn_selection", ctypes.c_int),
+        ("learn_selection", ctypes.c_int), ("fp16_intermediates", ctypes.c_int),
     ]
 
 
@@ -81,6 +81,7 @@ def make_cfg(config: dict) -> OrcCfg:
     c.cosine_kappa = float(mix.get("cosine_kappa", 2.18853))
     c.grad_clamp = float(config.get("loss", {}).get("grad_clamp", 150.0))
     c.clamp_scale = float(config.get("loss", {}).get("clamp_scale", 128.0))
+    c.fp16_intermediates = 1 if config.get("loss", {}).get("fp16_intermediates", False) else 0
     return c
 
 
@@ -118,6 +119,7 @@ class Oracle:
         L.orc_grid_layout.restype = ctypes.c_size_t
         L.orc_adam_step.argtypes = [ctypes.c_size_t, ctypes.c_size_t, _f, _f, _f, _f, _f, _f, _vp, _vp, _vp, _vp, _vp]
         L.orc_ema_step.argtypes = [ctypes.c_size_t, _f, ctypes.c_uint32, _vp, _vp]
+        L.orc_ema_step_half.argtypes = [ctypes.c_size_t, _f, ctypes.c_uint32, _vp, _vp]
         L.orc_train_step_f.argtypes = [_vp] * 9 + [ctypes.c_int, _f, _f, _f, _f, _f, _f, ctypes.c_int]
         L.orc_train_step_f.restype = _f
         L.orc_adjoint_f.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _f, _vp, _vp]
@@ -290,8 +292,10 @@ class Oracle:
                                          k(t["inputs"]), k(t["tgt_dir"]), k(t["tgt_rgb"]), k(t["tgt_pdf"]), n,
                                          loss_scale, lr, beta1, beta2, eps, l2, n_threads)
 
-    def ema_step(self, decay, step, weights, ema):
-        self.lib.orc_ema_step(weights.size, decay, step, _p(weights), _p(ema))
+    def ema_step(self, decay, step, weights, ema, half=False):
+        """half: tcnn's FP16 recurrence (ema_step_half_precision)."""
+        (self.lib.orc_ema_step_half if half else self.lib.orc_ema_step)(weights.size, decay, step, _p(weights),
+                                                                       _p(ema))
 
     def init_params(self, cfg, seed, passes=2):
         n_mlp, n_grid = self.n_params(cfg)

@@ -23,13 +23,15 @@ pytestmark = pytest.mark.gpu
 EMA_DECAY = 0.95
 
 
-def _cfg(prec):
+def _cfg(prec, full=None):
     import npmg
 
     c = npmg.benchmark_config("small", precision=prec)
     c["optimizer"] = {"otype": "Ema", "decay": EMA_DECAY,
                       "nested": {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99,
                                  "epsilon": 1e-15, "l2_reg": 1e-6}}
+    if full is not None:
+        c["optimizer"]["full_precision"] = full
     return c
 
 
@@ -62,12 +64,16 @@ def _gradient(rng, n_mlp, n, step):
     return g
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_adam_ema_three_steps_match_oracle(gpu, oracle, prec):
+@pytest.mark.parametrize("prec,full", [("fp32", None), ("bf16", None), ("fp16", None), ("fp32", False),
+                                       ("fp16", True)])
+def test_adam_ema_three_steps_match_oracle(gpu, oracle, prec, full):
+    """EMA in FP32, or in FP16 like tcnn's ema_step_half_precision: explicitly
+    ("full_precision": false) or by default where the parameters are FP16."""
     import torch
     import npmg
 
-    net = npmg.GuidingNetwork(_cfg(prec), seed=17)
+    half = full is False or (full is None and prec == "fp16")
+    net = npmg.GuidingNetwork(_cfg(prec, full), seed=17)
     n, n_mlp = net.n_params, net.n_mlp
     master = net.get_params(0).cpu().numpy().copy()
     ema = master.copy()
@@ -79,7 +85,7 @@ def test_adam_ema_three_steps_match_oracle(gpu, oracle, prec):
         g = _gradient(rng, n_mlp, n, t)
         net.optimizer_step(torch.from_numpy(g).cuda(), 128.0)
         oracle.adam_step(n_mlp, 128.0, 1e-2, 0.9, 0.99, 1e-15, 1e-6, master, g, m, v, steps)
-        oracle.ema_step(EMA_DECAY, t, master, ema)
+        oracle.ema_step(EMA_DECAY, t, master, ema, half=half)
         gm, gmm, gv, gs, ge, cur = _state(net)
         assert cur == t
         zero_grid = np.flatnonzero(g[n_mlp:] == 0) + n_mlp

@@ -42,3 +42,34 @@ def test_ema_debias(oracle):
     oracle.ema_step(d, 2, w2, ema)
     want = (w1 * d * (1 - d) + w2 * (1 - d)) / (1 - d * d)
     assert np.allclose(ema, want, rtol=1e-6)
+
+
+def test_half_ema_recurrence(oracle):
+    """tcnn's FP16 EMA (ema_step_half_precision, optimizers/ema.h:62-76): every
+    stored average is an FP16 value; the first step returns the FP16 weight
+    (debiasing); a weight within half an FP16 step of the average leaves it
+    unchanged (the stall the FP32 recurrence does not have); with weights held
+    constant at an FP16 value the average stays on it exactly."""
+    r = np.random.default_rng(5)
+    n = 50000
+    w = r.normal(0, 0.3, n).astype(np.float32)
+    ema = np.zeros(n, np.float32)
+    oracle.ema_step(0.95, 1, w, ema, half=True)
+    h = w.astype(np.float16).astype(np.float32)
+    assert np.array_equal(ema, h)
+    for t in range(2, 6):
+        w2 = (w + r.normal(0, 1e-2, n)).astype(np.float32)
+        oracle.ema_step(0.95, t, w2, ema, half=True)
+        assert np.array_equal(ema, ema.astype(np.float16).astype(np.float32))
+    # stall: 1 + 2^-11 is within half an FP16 step of 1 after the (1 - decay) weighting
+    e = np.ones(4, np.float32)
+    oracle.ema_step(0.95, 200, np.full(4, 1.0 + 2.0 ** -10, np.float32), e, half=True)
+    assert np.all(e == 1.0)
+    f = np.ones(4, np.float32)
+    oracle.ema_step(0.95, 200, np.full(4, 1.0 + 2.0 ** -10, np.float32), f, half=False)
+    assert np.all(f > 1.0)
+    # constant FP16 weights are a fixed point
+    c = h.copy()
+    for t in range(7, 12):
+        oracle.ema_step(0.95, t, h, c, half=True)
+    assert np.max(np.abs(c - h) / np.maximum(np.abs(h), 1e-3)) <= 2.0 ** -10

@@ -127,3 +127,42 @@ def test_clamp_follows_the_loss_scale(oracle):
     hi = 150.0 * 1e6
     assert np.sum(g128 == hi) > 0, "some entries must hit the clamp"
     assert np.allclose(g32k / 256.0, g128, rtol=1e-12, atol=1e-200)  # atol: FP64 subnormals
+
+
+def _h16(x):
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
+
+
+@pytest.mark.parametrize("numerics", ["reference", "stable"])
+def test_fp16_intermediates(oracle, numerics):
+    """The reference's __half adjoint buffers (train.cu:80-99, distribution.h:177-178):
+    every dL/draw is an FP16 value; the raw outputs enter rounded (rounding them
+    beforehand changes nothing); the result stays within a few FP16 steps of the
+    FP32 adjoint; and a gradient beyond the FP16 range becomes infinite and is
+    zeroed by the NaN test rather than clamped."""
+    K, n = 8, 512
+    base = dict(K=K, numerics=numerics, clamp=150.0)
+    c32 = _cfg(**base)
+    c16 = _cfg(**base)
+    c16.fp16_intermediates = 1
+    raw = _raw(K, n, 4, "reference").astype(np.float32)
+    td, tr, tp = _targets(n, 5)
+    g32, l32 = oracle.adjoint(c32, raw, td, tr, tp, 128.0)
+    g16, l16 = oracle.adjoint(c16, raw, td, tr, tp, 128.0)
+    assert np.array_equal(g16, _h16(g16))
+    g16b, _ = oracle.adjoint(c16, _h16(raw), td, tr, tp, 128.0)
+    assert np.array_equal(g16, g16b)
+    g32h, _ = oracle.adjoint(c32, _h16(raw), td, tr, tp, 128.0)
+    assert not np.array_equal(g16, g32h)  # the per-lobe dp rounding is a separate step
+    scale = np.abs(g32h).max(axis=0, keepdims=True) + 1e-30
+    assert np.max(np.abs(g16 - g32h) / scale) < 2.0 ** -8
+    assert np.allclose(l16, oracle.adjoint(c32, _h16(raw), td, tr, tp, 128.0)[1], rtol=0, atol=0)
+    # overflow: a loss scale that drives gradients past 65504 (no clamp)
+    c16.grad_clamp = 0.0
+    c32.grad_clamp = 0.0
+    big32, _ = oracle.adjoint(c32, _h16(raw), td, tr, tp, 2.0 ** 24)
+    big16, _ = oracle.adjoint(c16, raw, td, tr, tp, 2.0 ** 24)
+    over = np.abs(big32) > 65520.0 * (1 + 2.0 ** -8)
+    assert over.sum() > 0
+    assert np.all(big16[over] == 0)
+    assert np.all(np.isfinite(big16))

@@ -51,3 +51,24 @@ def test_atan2_and_special_values(oracle):
     tiny = np.array([1e-7, -3e-6, 2e-10], f32)
     assert np.allclose(oracle.math(3, tiny), np.expm1(tiny.astype(np.float64)), rtol=1e-6, atol=0)
     assert np.allclose(oracle.math(2, tiny), np.log1p(tiny.astype(np.float64)), rtol=1e-6, atol=0)
+
+
+def test_round_half_matches_numpy_float16(oracle):
+    """fn 12 (FP16 rounding, round to nearest even) against numpy's float16
+    conversion over normals, subnormals, ties, the overflow edge and specials."""
+    r = np.random.default_rng(12)
+    f32 = np.float32
+    mag = np.exp(r.uniform(np.log(1e-9), np.log(7e4), 200000))
+    x = (mag * np.sign(r.normal(size=mag.size))).astype(f32)
+    # exact ties between neighbouring FP16 values (normal and subnormal), both parities
+    h = np.arange(1, 0x7bff, 37, dtype=np.uint16).view(np.float16).astype(np.float64)
+    nxt = (np.arange(1, 0x7bff, 37, dtype=np.uint16) + 1).view(np.float16).astype(np.float64)
+    ties = ((h + nxt) / 2).astype(f32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, 65504.0, 65519.99, 65520.0, -65520.0, 2.0 ** -24,
+                        2.0 ** -25, 2.0 ** -25 * 1.0001, 2.0 ** -26, 6.1e-5, 6.103515625e-05, 3.4e38], f32)
+    x = np.concatenate([x, ties, -ties, special])
+    got = oracle.math(12, x)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).astype(f32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.isnan(oracle.math(12, np.array([np.nan], f32))[0])
