@@ -49,8 +49,9 @@ hipError_t launch_train_forward_mfma(const DevCfg& c, int precision, const void*
 hipError_t launch_train_backward_mfma(const DevCfg& c, int precision, const void* bfrag, const float* dout,
                                       const TrainMfmaWS& ws, float* denc, uint32_t n, uint32_t* overflow, int n_cu,
                                       hipStream_t s);
-hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& ws, float* partials,
-                             size_t partials_cap, float* dw, uint32_t n, uint32_t* overflow, hipStream_t s);
+hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& ws, const void* genc, const float* in,
+                             float* partials, size_t partials_cap, float* dw, uint32_t n, uint32_t* overflow,
+                             hipStream_t s);
 size_t wgrad_mfma_partials_needed(const DevCfg& c, uint32_t n);
 size_t train_mask_words(const DevCfg& c, uint32_t n);
 size_t bfrag_elements(const DevCfg& c);
@@ -777,7 +778,7 @@ struct npmg_ctx {
             sw = aux;
         }
         if (half())
-            hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, partials, partials_cap, dparams, n, of, sw),
+            hip_check(launch_wgrad_mfma(cfg.d, cfg.precision, mws, genc, in, partials, partials_cap, dparams, n, of, sw),
                       "weight gradient (mfma)");
         else
             hip_check(launch_wgrad_fp32(cfg.d, ws, partials, partials_cap, dparams, n, of, sw), "weight gradient");

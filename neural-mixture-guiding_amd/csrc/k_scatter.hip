@@ -5,13 +5,17 @@
 // Every (sample, level, corner) contributes w_k * dL/dy_{l,f} to one table
 // entry. The contributions are binned by (level, tile of 8192 / F entries),
 // so that one tile of gradients fits in LDS:
-//   1. count    per (bin, block of 2048 samples), histogram in LDS;
+//   1. count    per (bin, block of 512 samples), histogram in LDS;
 //   2. scan     per bin over its blocks, then over bins: each (bin, block)
 //               owns a contiguous run of records, in block order; the bins
 //               are cut into chunks of at most kChunk records;
 //   3. place    the records (local word index, F contributions; 1 + F
 //               words) at their slots (slot order inside a (bin, block) run
-//               is arbitrary), and max |dL/dy| of each level;
+//               is arbitrary), and max |dL/dy| of each level. For F = 2 the
+//               block's records are first sorted into their runs in LDS and
+//               then written run by run with consecutive lanes on
+//               consecutive records, instead of three scattered 4-byte stores
+//               per record;
 // Steps 1-2 read only the sample positions, so the training step runs them
 // (launch_grid_scatter_prepare) beside the forward pass, and only 3-4 wait
 // for dL/dy (launch_grid_scatter_finish).
@@ -45,7 +49,7 @@ namespace {
 
 using namespace dev;
 
-constexpr uint32_t kSamplesPerBlock = 2048;
+constexpr uint32_t kSamplesPerBlock = 512;
 constexpr uint32_t kTileWords = 8192;  // words of one tile: tile entries x F
 constexpr uint32_t kChunk = 65536;     // records reduced by one workgroup
 constexpr uint32_t kMaxBinsPerLevel = 8192;
@@ -55,12 +59,14 @@ struct BinLayout {
     uint32_t shift;        // log2(tile entries)
     uint32_t nblk;         // sample blocks
     uint32_t max_chunks;   // capacity of the chunk table
+    uint32_t bin_off_max;  // bins of the largest level
 };
 
 struct Scratch {
     uint32_t* counts;      // [bins][nblk] counts, then exclusive offsets within the bin
     uint32_t* bin_start;   // [bins] first record of the bin
     uint32_t* bin_chunks;  // [bins]
+    uint32_t* bin_count;   // [bins] records of the bin
     uint4* chunks;         // [max_chunks] (bin, begin, end, chunks of the bin); chunks[-1].x = count
     uint32_t* maxbits;     // [L] max |dL/dy| of each level (float bits)
     uint32_t* rec;         // [R][1 + F] records: word index within the tile, then F contributions
@@ -177,6 +183,7 @@ __global__ void __launch_bounds__(256) k_bin_chunks(const BinLayout b, uint32_t 
         const uint32_t k = (t + kChunk - 1u) / kChunk;
         sc.bin_start[i] = rec;
         sc.bin_chunks[i] = k;
+        sc.bin_count[i] = t;
         for (uint32_t j = 0; j < k; ++j) {
             const uint32_t be = rec + j * kChunk;
             const uint32_t en = be + kChunk < rec + t ? be + kChunk : rec + t;
@@ -192,6 +199,7 @@ __global__ void __launch_bounds__(256) k_bin_chunks(const BinLayout b, uint32_t 
 // overflow flag then skips the step under dynamic loss scaling).
 __device__ __forceinline__ float contrib(float w, float d) { return __builtin_isfinite(d) ? w * d : 0.0f; }
 
+// Place for F != 2: each record stored where its bin's cursor points.
 __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayout b, const float* __restrict__ in,
                                                    const float* __restrict__ dgrid, Scratch sc, uint32_t n,
                                                    uint32_t* __restrict__ overflow) {
@@ -217,39 +225,15 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
     for (int i = 0; i < (int)kPerThread; ++i) {
         const uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
         if (q >= n) break;
-        uint32_t idx[8], slot[8];
+        uint32_t idx[8];
         float w[8];
         const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
-        if (F == 2) {
-            const float d0 = dgrid[(size_t)(2 * l) * n + q], d1 = dgrid[(size_t)(2 * l + 1) * n + q];
-            seen(d0);
-            seen(d1);
-            if (nc == 8) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) slot[k] = atomicAdd(&cursor[idx[k] >> b.shift], 1u);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    uint32_t* r = sc.rec + (size_t)slot[k] * 3u;
-                    r[0] = (idx[k] & mask) * 2u;
-                    r[1] = __float_as_uint(contrib(w[k], d0));
-                    r[2] = __float_as_uint(contrib(w[k], d1));
-                }
-            } else {
-                for (int k = 0; k < nc; ++k) {
-                    uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * 3u;
-                    r[0] = (idx[k] & mask) * 2u;
-                    r[1] = __float_as_uint(contrib(w[k], d0));
-                    r[2] = __float_as_uint(contrib(w[k], d1));
-                }
-            }
-        } else {
-            for (int f = 0; f < F; ++f) seen(dgrid[(size_t)(l * F + f) * n + q]);
-            for (int k = 0; k < nc; ++k) {
-                uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * (1u + F);
-                r[0] = (idx[k] & mask) * (uint32_t)F;
-                for (int f = 0; f < F; ++f)
-                    r[1 + f] = __float_as_uint(contrib(w[k], dgrid[(size_t)(l * F + f) * n + q]));
-            }
+        for (int f = 0; f < F; ++f) seen(dgrid[(size_t)(l * F + f) * n + q]);
+        for (int k = 0; k < nc; ++k) {
+            uint32_t* r = sc.rec + (size_t)atomicAdd(&cursor[idx[k] >> b.shift], 1u) * (1u + F);
+            r[0] = (idx[k] & mask) * (uint32_t)F;
+            for (int f = 0; f < F; ++f)
+                r[1 + f] = __float_as_uint(contrib(w[k], dgrid[(size_t)(l * F + f) * n + q]));
         }
     }
     // max |dL/dy| of the block, then one global atomic per block (bits of
@@ -263,9 +247,107 @@ __global__ void __launch_bounds__(256) k_bin_place(const DevCfg c, const BinLayo
     if (bad && overflow) atomicOr(overflow, 1u);
 }
 
+// Records of one block for F = 2 (at most 8 per sample).
+constexpr uint32_t kStage = kSamplesPerBlock * 8u;
+
+// Place for F = 2, staged: the block's count in each bin follows from the
+// scanned offsets (the next block's offset, or the bin's total), so the
+// block's records can be sorted into their runs in LDS as they are made:
+// position loc[bin] + arrival within the bin. Position r of the staged block
+// then goes to global slot run[bin] + r - loc[bin], so consecutive lanes
+// write consecutive records of a run.
+__global__ void __launch_bounds__(256) k_bin_place2(const DevCfg c, const BinLayout b, const float* __restrict__ in,
+                                                    const float* __restrict__ dgrid, Scratch sc, uint32_t n,
+                                                    uint32_t* __restrict__ overflow) {
+    extern __shared__ uint32_t sm[];
+    __shared__ float wave_max[4];
+    __shared__ uint32_t red[256];
+    const int l = blockIdx.y;
+    const uint32_t b0 = b.bin_off[l], nb = b.bin_off[l + 1] - b0;
+    const uint32_t nbp = (b.bin_off_max + 3u) & ~3u;  // the arrays' stride: the largest level's bins
+    uint32_t* cur = sm;             // [nb] local cursor of each bin, from loc[bin]
+    uint32_t* off = sm + nbp;       // [nb] run[bin] - loc[bin]
+    uint32_t* tw = sm + 2u * nbp;   // [kStage] bin << 16 | word index
+    float* s0 = reinterpret_cast<float*>(tw + kStage);
+    float* s1 = s0 + kStage;
+    const uint32_t blk = blockIdx.x;
+    // positions and dL/dy first: their latency overlaps the prologue below
+    const int g = c.enc_begin[c.grid_enc];
+    float px[kPerThread], py[kPerThread], pz[kPerThread], d0[kPerThread], d1[kPerThread];
+    load_positions(in, g, n, px, py, pz);
+#pragma unroll
+    for (int i = 0; i < (int)kPerThread; ++i) {
+        uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
+        q = q < n ? q : n - 1u;
+        d0[i] = dgrid[(size_t)(2 * l) * n + q];
+        d1[i] = dgrid[(size_t)(2 * l + 1) * n + q];
+    }
+    // the block's count per bin and their exclusive scan over the level's bins
+    const uint32_t per = (nb + 255u) / 256u;
+    const uint32_t lo = threadIdx.x * per, hi = lo + per < nb ? lo + per : nb;
+    uint32_t sum = 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t* offs = sc.counts + (size_t)(b0 + i) * b.nblk;
+        const uint32_t cnt = (blk + 1u < b.nblk ? offs[blk + 1u] : sc.bin_count[b0 + i]) - offs[blk];
+        sum += cnt;
+    }
+    uint32_t run;
+    block_scan(sum, red, run);
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t* offs = sc.counts + (size_t)(b0 + i) * b.nblk;
+        const uint32_t cnt = (blk + 1u < b.nblk ? offs[blk + 1u] : sc.bin_count[b0 + i]) - offs[blk];
+        cur[i] = run;
+        off[i] = sc.bin_start[b0 + i] + offs[blk] - run;
+        run += cnt;
+    }
+    __syncthreads();
+    const uint32_t mask = (1u << b.shift) - 1u;
+    float mx = 0.0f;
+    bool bad = false;
+    auto seen = [&](float d) {
+        if (__builtin_isfinite(d)) mx = smax(mx, fabsf(d));
+        else bad = true;
+    };
+#pragma unroll
+    for (int i = 0; i < (int)kPerThread; ++i) {
+        const uint32_t q = blockIdx.x * kSamplesPerBlock + (uint32_t)i * 256u + threadIdx.x;
+        if (q >= n) break;
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, px[i], py[i], pz[i], idx, w);
+        seen(d0[i]);
+        seen(d1[i]);
+        for (int k = 0; k < nc; ++k) {
+            const uint32_t bin = idx[k] >> b.shift;
+            const uint32_t p = atomicAdd(&cur[bin], 1u);
+            tw[p] = (bin << 16) | ((idx[k] & mask) * 2u);
+            s0[p] = contrib(w[k], d0[i]);
+            s1[p] = contrib(w[k], d1[i]);
+        }
+    }
+    __syncthreads();
+    // every bin's cursor now ends its range: the last one ends the block's records
+    const uint32_t total = nb ? cur[nb - 1u] : 0u;
+    for (uint32_t r = threadIdx.x; r < total; r += 256u) {
+        const uint32_t t = tw[r];
+        reinterpret_cast<uint3*>(sc.rec)[off[t >> 16] + r] =
+            make_uint3(t & 0xffffu, __float_as_uint(s0[r]), __float_as_uint(s1[r]));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = smax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63u) == 0) wave_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(&sc.maxbits[l], __float_as_uint(smax(smax(wave_max[0], wave_max[1]), smax(wave_max[2], wave_max[3]))));
+    if (bad && overflow) atomicOr(overflow, 1u);
+}
+
+// 512 threads per workgroup: the 64 KB tile allows two workgroups per CU,
+// and the record loads are latency bound, so more of them are kept in flight.
+constexpr uint32_t kReduceThreads = 512;
+
 template <bool DET>
-__global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLayout b, Scratch sc,
-                                                    float* __restrict__ grad, uint32_t n) {
+__global__ void __launch_bounds__(kReduceThreads) k_bin_reduce(const DevCfg c, const BinLayout b, Scratch sc,
+                                                               float* __restrict__ grad, uint32_t n) {
     using Acc = unsigned long long;
     __shared__ Acc tile[kTileWords];
     if (blockIdx.x >= sc.chunks[-1].x) return;
@@ -278,36 +360,31 @@ __global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLay
     const uint32_t entries = size - e0 < (1u << b.shift) ? size - e0 : (1u << b.shift);
     const uint32_t words = entries * (uint32_t)F;
     const int s = fixed_shift(sc.maxbits[l], n * 8u);
-    for (uint32_t i = threadIdx.x; i < kTileWords; i += 256u) tile[i] = (Acc)0;
+    for (uint32_t i = threadIdx.x; i < kTileWords; i += kReduceThreads) tile[i] = (Acc)0;
     __syncthreads();
     auto add = [&](uint32_t wi, float v) { atomicAdd(&tile[wi], (Acc)to_fixed(v, s)); };
     if (F == 2) {
         // 8 records per thread in flight: all loads, then the LDS atomics.
         constexpr uint32_t U = 8;
+        const uint3* rec = reinterpret_cast<const uint3*>(sc.rec);
         uint32_t r0 = ch.y + threadIdx.x;
-        for (; r0 + 256u * (U - 1u) < ch.z; r0 += 256u * U) {
-            uint32_t wi[U];
-            float v0[U], v1[U];
+        for (; r0 + kReduceThreads * (U - 1u) < ch.z; r0 += kReduceThreads * U) {
+            uint3 rv[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) rv[u] = rec[r0 + kReduceThreads * u];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
-                const uint32_t* rp = sc.rec + (size_t)(r0 + 256u * u) * 3u;
-                wi[u] = rp[0];
-                v0[u] = __uint_as_float(rp[1]);
-                v1[u] = __uint_as_float(rp[2]);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) {
-                add(wi[u], v0[u]);
-                add(wi[u] + 1u, v1[u]);
+                add(rv[u].x, __uint_as_float(rv[u].y));
+                add(rv[u].x + 1u, __uint_as_float(rv[u].z));
             }
         }
-        for (; r0 < ch.z; r0 += 256u) {
-            const uint32_t* rp = sc.rec + (size_t)r0 * 3u;
-            add(rp[0], __uint_as_float(rp[1]));
-            add(rp[0] + 1u, __uint_as_float(rp[2]));
+        for (; r0 < ch.z; r0 += kReduceThreads) {
+            const uint3 r = rec[r0];
+            add(r.x, __uint_as_float(r.y));
+            add(r.x + 1u, __uint_as_float(r.z));
         }
     } else {
-        for (uint32_t r = ch.y + threadIdx.x; r < ch.z; r += 256u) {
+        for (uint32_t r = ch.y + threadIdx.x; r < ch.z; r += kReduceThreads) {
             const uint32_t* rp = sc.rec + (size_t)r * (1u + F);
             for (int f = 0; f < F; ++f) add(rp[0] + f, __uint_as_float(rp[1 + f]));
         }
@@ -315,9 +392,9 @@ __global__ void __launch_bounds__(256) k_bin_reduce(const DevCfg c, const BinLay
     __syncthreads();
     const size_t gbase = ((size_t)c.offsets[l] + e0) * F;
     if (ch.w == 1u) {
-        for (uint32_t i = threadIdx.x; i < words; i += 256u) grad[gbase + i] = ldexpf((float)(long long)tile[i], -s);
+        for (uint32_t i = threadIdx.x; i < words; i += kReduceThreads) grad[gbase + i] = ldexpf((float)(long long)tile[i], -s);
     } else {
-        for (uint32_t i = threadIdx.x; i < words; i += 256u) {
+        for (uint32_t i = threadIdx.x; i < words; i += kReduceThreads) {
             if (tile[i] == (Acc)0) continue;
             if (DET) atomicAdd(&sc.wide[gbase + i], tile[i]);
             else atomicAdd(&grad[gbase + i], ldexpf((float)(long long)tile[i], -s));
@@ -361,6 +438,7 @@ BinLayout make_layout(const DevCfg& c, uint32_t n, uint32_t* n_bins) {
         bins += (size + (1u << b.shift) - 1u) >> b.shift;
     }
     b.bin_off[c.n_levels] = bins;
+    for (int l = 0; l < c.n_levels; ++l) b.bin_off_max = std::max(b.bin_off_max, b.bin_off[l + 1] - b.bin_off[l]);
     b.nblk = (n + kSamplesPerBlock - 1u) / kSamplesPerBlock;
     const size_t recs = (size_t)n * c.n_levels * 8u;
     b.max_chunks = (uint32_t)(recs / kChunk + bins + 1u);
@@ -388,6 +466,7 @@ Scratch carve(const DevCfg& c, const BinLayout& b, uint32_t n_bins, uint32_t n, 
     sc.counts = cv.take<uint32_t>((size_t)n_bins * b.nblk);
     sc.bin_start = cv.take<uint32_t>(n_bins);
     sc.bin_chunks = cv.take<uint32_t>(n_bins);
+    sc.bin_count = cv.take<uint32_t>(n_bins);
     uint4* ct = cv.take<uint4>((size_t)b.max_chunks + 1u);
     sc.chunks = ct ? ct + 1 : nullptr;
     sc.maxbits = cv.take<uint32_t>(32);
@@ -469,12 +548,21 @@ hipError_t launch_grid_scatter_finish(const DevCfg& c, const float* in, const fl
     e = scatter_setup(c, n, deterministic, scratch, scratch_bytes, &b, &n_bins, &sc);
     if (e != hipSuccess) return e;
     const dim3 grid(b.nblk, c.n_levels);
-    hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_bytes(c, b), s, c, b, in, dgrid, sc, n, overflow);
+    if (c.F == 2) {
+        const size_t lds = (2u * (size_t)((b.bin_off_max + 3u) & ~3u) + 3u * kStage) * 4u;
+        if (lds > 65536u) {
+            e = hipFuncSetAttribute((const void*)k_bin_place2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_bin_place2, grid, dim3(256), lds, s, c, b, in, dgrid, sc, n, overflow);
+    } else {
+        hipLaunchKernelGGL(k_bin_place, grid, dim3(256), hist_bytes(c, b), s, c, b, in, dgrid, sc, n, overflow);
+    }
     if (deterministic) {
-        hipLaunchKernelGGL(k_bin_reduce<true>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
+        hipLaunchKernelGGL(k_bin_reduce<true>, dim3(b.max_chunks), dim3(kReduceThreads), 0, s, c, b, sc, grad, n);
         hipLaunchKernelGGL(k_bin_finish, dim3(n_bins), dim3(256), 0, s, c, b, sc, grad, n);
     } else {
-        hipLaunchKernelGGL(k_bin_reduce<false>, dim3(b.max_chunks), dim3(256), 0, s, c, b, sc, grad, n);
+        hipLaunchKernelGGL(k_bin_reduce<false>, dim3(b.max_chunks), dim3(kReduceThreads), 0, s, c, b, sc, grad, n);
     }
     return hipGetLastError();
 }

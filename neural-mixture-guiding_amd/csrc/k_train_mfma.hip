@@ -13,8 +13,10 @@
 //             gradient feeds the next backward step from registers; ReLU'
 //             comes from the stored sign bits (fully_fused_mlp.cu:150-259).
 // Stored for the weight gradients (operand type, [dim][n]): layer inputs and
-// output gradients of every layer; dL/d(grid features) [L F][n] in FP32 for the
-// grid scatter.
+// output gradients of every layer (the layer-0 inputs only outside the "grid,
+// then OneBlob(16)" layout with the fast numerics, whose inputs the weight
+// gradient rebuilds); dL/d(grid features) [L F][n] in FP32 for the grid
+// scatter.
 #include <type_traits>
 
 #include "device.h"
@@ -117,14 +119,19 @@ __global__ void __launch_bounds__(256) k_train_fwd_mfma(const DevCfg c, const T*
                 ba = mf::pack8<T>(va);
                 bb = mf::pack8<T>(vb);
             }
+            // The grid and OneBlob K-blocks are not stored: the weight gradient
+            // rebuilds them, bit for bit, from the level-major features and the
+            // input records (k_wgrad_mfma).
             const int f0 = 8 * (2 * kb + h);
+            if (!ob16) {
 #pragma unroll
-            for (int p = 0; p < NP; ++p)
+                for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    if (oka) x0[p * px0 + (size_t)(f0 + e) * n + qra] = mf::get<T>(ba, p, e);
-                    if (okb) x0[p * px0 + (size_t)(f0 + e) * n + qrb] = mf::get<T>(bb, p, e);
-                }
+                    for (int e = 0; e < 8; ++e) {
+                        if (oka) x0[p * px0 + (size_t)(f0 + e) * n + qra] = mf::get<T>(ba, p, e);
+                        if (okb) x0[p * px0 + (size_t)(f0 + e) * n + qrb] = mf::get<T>(bb, p, e);
+                    }
+            }
 #pragma unroll
             for (int t = 0; t < TH; ++t) {
                 const V a = mf::load_frag<T>(sw, t * KB0 + kb, lane);
@@ -340,111 +347,6 @@ __global__ void __launch_bounds__(256) k_train_bwd_mfma(const DevCfg c, const T*
     if (overflow && !__all(finite ? 1 : 0) && lane == 0) atomicOr(overflow, 1u);
 }
 
-// Weight gradients on MFMA: dW_l[o][i] = sum_q dY_l[o][q] X_l[i][q] with the
-// batch as the K dimension. Block = (layer, batch split), wave = a quarter of
-// the split's samples; a wave accumulates every 32 x 32 tile of two output
-// tiles at once (up to 2 x 4 tiles, 128 accumulators), so each 16-sample step
-// loads its dY and X fragments once: 8 consecutive samples of one row per
-// lane, i.e. 16-byte loads from the [dim][n] buffers. Each wave writes its own
-// partial row; a deterministic reduction over the rows follows.
-template <class T>
-__global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainMfmaWS ws, float* __restrict__ partials,
-                                                    uint32_t n, uint32_t q_per_wave, uint32_t n_mlp) {
-    using V = typename mf::Frag<T>::vec;
-    using E = typename mf::Op<T>::elem;
-    constexpr int TI = 4;  // input tiles per pass
-    const int l = blockIdx.x;
-    const int H = c.n_layers - 1;
-    const int iw = c.layer_in[l], ow = c.layer_out[l];
-    const E* X = l == 0 ? static_cast<const E*>(ws.x0) : static_cast<const E*>(ws.xh) + (size_t)(l - 1) * c.width * n;
-    const E* G = l == H ? static_cast<const E*>(ws.dyo) : static_cast<const E*>(ws.dyh) + (size_t)l * c.width * n;
-    // plane strides of the split type: the buffers' sizes
-    const size_t px = (l == 0 ? (size_t)c.enc_padded : (size_t)H * c.width) * n;
-    const size_t pg = (l == H ? (size_t)c.n_out_pad : (size_t)H * c.width) * n;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int h = lane >> 5, j = lane & 31;
-    const uint32_t row = blockIdx.y * 4u + (uint32_t)wave;
-    const uint32_t qb = row * q_per_wave;
-    const uint32_t qe = min(n, qb + q_per_wave);
-    const int n_to = ow / 32, n_ti = (iw + 31) / 32;
-    float* P = partials + (size_t)row * n_mlp + c.layer_off[l];
-    // 16-byte fragment loads need 16-byte aligned rows: n a multiple of 8.
-    const bool vec = (n & 7u) == 0u;
-    for (int to0 = 0; to0 < n_to; to0 += 2) {
-        const int nto = n_to - to0 < 2 ? n_to - to0 : 2;
-        for (int ti0 = 0; ti0 < n_ti; ti0 += TI) {
-            mf::f32x16 acc[2][TI];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < TI; ++b) acc[a][b] = mf::f32x16{};
-            const E* ga[2];
-            const E* xb[TI];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) ga[a] = G + (size_t)(32 * (to0 + (a < nto ? a : 0)) + j) * n;
-#pragma unroll
-            for (int b = 0; b < TI; ++b) {
-                // rows beyond iw (a partial or absent tile) read row iw - 1 and are not stored
-                const int i = 32 * (ti0 + b) + j;
-                xb[b] = X + (size_t)(i < iw ? i : iw - 1) * n;
-            }
-            auto step = [&](uint32_t q, bool full) {
-                V av[2], bv[TI];
-#pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    if (full) {
-                        av[a] = mf::load_rows16<T>(ga[a] + q + 8u * h, pg);
-                    } else {
-#pragma unroll
-                        for (int p = 0; p < mf::Op<T>::planes; ++p)
-#pragma unroll
-                            for (int e = 0; e < 8; ++e) {
-                                const uint32_t qq = q + 8u * h + e;
-                                mf::set<T>(av[a], p, e, qq < qe ? ga[a][p * pg + qq] : (E)0.0f);
-                            }
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < TI; ++b) {
-                    if (full) {
-                        bv[b] = mf::load_rows16<T>(xb[b] + q + 8u * h, px);
-                    } else {
-#pragma unroll
-                        for (int p = 0; p < mf::Op<T>::planes; ++p)
-#pragma unroll
-                            for (int e = 0; e < 8; ++e) {
-                                const uint32_t qq = q + 8u * h + e;
-                                mf::set<T>(bv[b], p, e, qq < qe ? xb[b][p * px + qq] : (E)0.0f);
-                            }
-                    }
-                }
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int b = 0; b < TI; ++b) acc[a][b] = mf::Frag<T>::mma(av[a], bv[b], acc[a][b]);
-            };
-            uint32_t q = qb;
-            if (vec)
-                for (; q + 16u <= qe; q += 16u) step(q, true);
-            for (; q < qe; q += 16u) step(q, false);
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                if (a >= nto) break;
-#pragma unroll
-                for (int b = 0; b < TI; ++b) {
-                    const int i = 32 * (ti0 + b) + j;
-                    if (i >= iw) continue;
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int o = 32 * (to0 + a) + 8 * (r >> 2) + 4 * h + (r & 3);
-                        P[(size_t)o * iw + i] = acc[a][b][r];
-                    }
-                }
-            }
-        }
-    }
-}
-
 // Transposed weight fragments for the backward chain: fragment (layer, t, kb)
 // over input rows 32 t .. 32 t + 31 and output K-block kb; lane (h, i),
 // element e holds W_l[o][32 t + i] with o = 16 kb + 8 h + e on the output
@@ -613,46 +515,6 @@ hipError_t launch_train_backward_mfma(const DevCfg& c, int precision, const void
 
 size_t train_mask_words(const DevCfg& c, uint32_t n) {
     return (size_t)((n + 63u) / 64u) * (size_t)c.n_hidden * (size_t)(c.width / 32) * 64u;
-}
-
-namespace {
-// Batch splitting of the MFMA weight gradient: one partial row per wave,
-// 4 waves per block, at least 1024 samples per wave and at most 1024 rows.
-// Partial rows of the weight gradient: one wave per row and layer, about 512
-// samples each (2 waves per SIMD at 2^18 samples; the rows are summed by
-// k_reduce_splits).
-void wgrad_split(uint32_t n, uint32_t* rows, uint32_t* q_per_wave) {
-    uint32_t r = (n + 511u) / 512u;
-    r = (r + 3u) / 4u * 4u;
-    if (r > 1024u) r = 1024u;
-    if (r < 4u) r = 4u;
-    *rows = r;
-    *q_per_wave = ((n + r - 1u) / r + 15u) / 16u * 16u;
-}
-}  // namespace
-
-hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& ws, float* partials,
-                             size_t partials_cap, float* dw, uint32_t n, uint32_t* overflow, hipStream_t s) {
-    const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
-    uint32_t rows, qpw;
-    wgrad_split(n, &rows, &qpw);
-    if ((size_t)rows * n_mlp > partials_cap) return hipErrorInvalidValue;
-    const dim3 grid(c.n_layers, rows / 4u);
-    if (precision == PREC_BF16) hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
-    else if (precision == PREC_FP16) hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
-    else if (precision == PREC_BF16X3)
-        hipLaunchKernelGGL((k_wgrad_mfma<mf::Bf16x3>), grid, dim3(256), 0, s, c, ws, partials, n, qpw, n_mlp);
-    else return hipErrorInvalidValue;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_reduce_splits(partials, rows, n_mlp, dw, s, overflow);
-}
-
-size_t wgrad_mfma_partials_needed(const DevCfg& c, uint32_t n) {
-    const uint32_t n_mlp = c.layer_off[c.n_layers - 1] + (uint32_t)(c.layer_in[c.n_layers - 1] * c.layer_out[c.n_layers - 1]);
-    uint32_t rows, qpw;
-    wgrad_split(n, &rows, &qpw);
-    return (size_t)rows * n_mlp;
 }
 
 size_t bfrag_elements(const DevCfg& c) { return (size_t)n_bfrag_of(c) * 512u; }
