@@ -265,6 +265,8 @@ struct npmg_ctx {
     uint32_t gl_n = 0;
     float* ema_pow = nullptr;
     uint32_t ema_pow_len = 0;
+    float* adam_bias = nullptr;  // Adam's bias correction per step count (k_adam)
+    static constexpr uint32_t kAdamBiasLen = 1u << 16;
 
     const float* infer_params() const { return ema ? ema : master; }
 
@@ -289,7 +291,7 @@ struct npmg_ctx {
                         (void*)wt_infer, (void*)ws.enc, (void*)ws.acts, (void*)ws.g, (void*)ws.dout, (void*)ws.loss,
                         (void*)partials, frag_train, frag_infer, htab_train, htab_infer, (void*)ws.dgrid,
                         scatter_scratch, genc, bfrag_train, mws.x0, mws.xh, mws.dyh, mws.dyo, (void*)mws.mask,
-                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)mean_partials, (void*)gl_nodes,
+                        (void*)mws.raw, (void*)sstate, (void*)ema_pow, (void*)adam_bias, (void*)mean_partials, (void*)gl_nodes,
                         (void*)queue_counter, (void*)fp8_amax, (void*)fp8_scales, (void*)pack_scratch,
                         (void*)stage})
             if (p) (void)hipFree(p);
@@ -824,6 +826,8 @@ struct npmg_ctx {
         a.st = sstate;
         a.ema_pow = ema_pow;
         a.ema_pow_len = ema_pow_len;
+        a.adam_bias = adam_bias;
+        a.adam_bias_len = kAdamBiasLen;
         a.lr = cfg.lr;
         a.beta1 = cfg.beta1;
         a.beta2 = cfg.beta2;
@@ -977,6 +981,9 @@ npmg_status npmg_create(const char* json_config, uint64_t seed, npmg_ctx** out) 
         ctx->ema_pow_len = (uint32_t)pw.size();
         ctx->ema_pow = dalloc<float>(pw.size());
         hip_check(hipMemcpy(ctx->ema_pow, pw.data(), pw.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+        ctx->adam_bias = dalloc<float>(npmg_ctx::kAdamBiasLen);
+        hip_check(launch_adam_bias_table(c.beta1, c.beta2, ctx->adam_bias, npmg_ctx::kAdamBiasLen, nullptr),
+                  "adam bias table");
         hip_check(hipMemset(ctx->m, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->v, 0, c.n_params * sizeof(float)), "memset");
         hip_check(hipMemset(ctx->steps, 0, c.n_params * sizeof(uint32_t)), "memset");

@@ -265,6 +265,22 @@ __global__ void __launch_bounds__(256) k_reduce_splits(const float* __restrict__
     }
 }
 
+// Adam's bias correction at step t (optimizers/adam.h:88-90):
+// sqrt(1 - beta2^t) / (1 - beta1^t), beta^t as exp(t log beta) in FP32.
+__device__ __forceinline__ float adam_bias(float beta1, float beta2, uint32_t t) {
+    const float b2t = fm::exp((float)t * fm::log(beta2)), b1t = fm::exp((float)t * fm::log(beta1));
+    return fm::sqrt_(1 - b2t) / (1 - b1t);
+}
+
+// The bias correction of every step below len, tabulated once per context: it
+// depends on the step count alone, and the two exponentials and the logarithm
+// per parameter were most of the optimiser's arithmetic.
+__global__ void __launch_bounds__(256) k_adam_bias_table(float beta1, float beta2, float* __restrict__ out,
+                                                         uint32_t len) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < len) out[t] = adam_bias(beta1, beta2, t);
+}
+
 // tiny-cuda-nn optimizers/adam.h:47-119 followed by the EMA of
 // optimizers/ema.h:62-76 (full-precision EMA).
 __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restrict__ master,
@@ -285,8 +301,7 @@ __global__ void __launch_bounds__(256) k_adam(const AdamArgs a, float* __restric
         m[i] = m1;
         v[i] = v1;
         const uint32_t t = ++steps[i];
-        const float b2t = fm::exp((float)t * fm::log(a.beta2)), b1t = fm::exp((float)t * fm::log(a.beta1));
-        const float lr_t = a.lr * (fm::sqrt_(1 - b2t) / (1 - b1t));
+        const float lr_t = a.lr * (t < a.adam_bias_len ? a.adam_bias[t] : adam_bias(a.beta1, a.beta2, t));
         const float eff = smin(smax(lr_t / (fm::sqrt_(v1) + a.eps), 0.0f), 3.402823466e38f);
         w = w - eff * m1;
         master[i] = w;
@@ -494,6 +509,12 @@ hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, floa
                        float* ema, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a, master, grad, m, v, steps, ema);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam_bias_table(float beta1, float beta2, float* out, uint32_t len, hipStream_t s) {
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adam_bias_table, dim3((len + 255u) / 256u), dim3(256), 0, s, beta1, beta2, out, len);
     return hipGetLastError();
 }
 

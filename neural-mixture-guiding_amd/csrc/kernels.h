@@ -194,6 +194,10 @@ struct AdamArgs {
     const ScaleState* st;
     const float* ema_pow;
     uint32_t ema_pow_len;
+    // Adam's bias correction sqrt(1 - beta2^t) / (1 - beta1^t) for t < adam_bias_len
+    // (launch_adam_bias_table; later steps compute it)
+    const float* adam_bias;
+    uint32_t adam_bias_len;
     // operand-type copies of the grid table (parameters n_matrix..n) written in
     // the same pass: precision PREC_BF16 / PREC_FP16, or none for PREC_FP32
     int half_precision;
@@ -202,6 +206,9 @@ struct AdamArgs {
 };
 hipError_t launch_adam(const AdamArgs& a, float* master, const float* grad, float* m, float* v, uint32_t* steps,
                        float* ema, hipStream_t s);
+// out[t] = sqrt(1 - beta2^t) / (1 - beta1^t) for t in [0, len), with the
+// operations k_adam uses, so a table entry equals the value computed in place.
+hipError_t launch_adam_bias_table(float beta1, float beta2, float* out, uint32_t len, hipStream_t s);
 // After the optimiser: counts the step and, with dynamic scaling, halves the
 // scale on overflow or doubles it after `window` finite steps; clears the flag.
 hipError_t launch_scale_update(ScaleState* st, bool dynamic, uint32_t window, float max_scale, hipStream_t s);
