@@ -290,29 +290,82 @@ def main():
         log(f"{precision}: held-out loss after {steps} steps {value:.4f}")
         return value
 
-    def pdf_agreement(make_cfg, precision, cosine, train_steps=200, n=1 << 16):
-        """Relative pdf difference of the `precision` path against the FP32 path on
-        identical parameters (an FP32 network trained for `train_steps` steps on
-        the analytic field, copied into the other): median, 99th percentile, max
-        over the 4 evaluation pdfs of n queries."""
+    def pdf_agreement(make_cfg, precisions, cosine, train_steps=200, n=1 << 16, n_grad=1 << 14):
+        """Each precision against the FP32 path on identical parameters (an FP32
+        network trained for `train_steps` steps on the analytic field, copied
+        into the other; the FP32 path is within 1e-5 of FP64 on the same
+        parameters, so these are also the errors against FP64):
+          pdf: relative difference of the 4 evaluation pdfs of n queries
+               (median, 99th percentile, max);
+          mixture: the mixture parameters the query kernel's raw outputs map to
+               (softmax mapping, in FP64: lobe weights, absolute; concentrations,
+               relative; mean directions, angle in radians), median and p99;
+          gradient: dL/dparams of n_grad training records (the library's
+               forward, adjoint and backward in that precision), relative
+               2-norm error of the MLP block and of the grid block.
+        FP8 is inference only: no gradient."""
         ref = npmg.GuidingNetwork(make_cfg("fp32"), seed=7272)
         batch = npmg.synth_train(123, N_TRAIN)
         for k in range(train_steps):
             lib.npmg_synth_train(123, k * N_TRAIN, N_TRAIN, P(batch["inputs"]), P(batch["tgt_dir"]),
                                  P(batch["tgt_rgb"]), P(batch["tgt_pdf"]), stream.cuda_stream)
             ref.train_step_graph(batch["inputs"], batch["tgt_dir"], batch["tgt_rgb"], batch["tgt_pdf"], loss)
-        other = npmg.GuidingNetwork(make_cfg(precision), seed=7272)
-        other.set_params(ref.get_params(1))
-        ref.set_params(ref.get_params(1))
+        params = ref.get_params(1)
+        ref.set_params(params)
         qq = npmg.synth_queries(555, n)
         nrm = qq["normal"] if cosine else None
-        a = ref.sample_pdf(qq["inputs"], nrm, None, qq["eval_dirs"], cosine_aware=cosine)["eval_pdf"]
-        b = other.sample_pdf(qq["inputs"], nrm, None, qq["eval_dirs"], cosine_aware=cosine)["eval_pdf"]
-        rel = ((b.double() - a.double()).abs() / a.double()).flatten()
+        tt = npmg.synth_train(777, n_grad)
+        mix_ok = make_cfg("fp32")["mixture"].get("otype", "vMF") == "vMF" and \
+            make_cfg("fp32")["mixture"].get("mapping", "softmax") == "softmax"
+
+        def run(net, grads):
+            o = net.sample_pdf(qq["inputs"], nrm, None, qq["eval_dirs"], cosine_aware=cosine, raw=True)
+            g = None
+            if grads:
+                raw = net.forward(tt["inputs"])
+                dout, _ = net.mixture_adjoint(raw, tt["tgt_dir"], tt["tgt_rgb"], tt["tgt_pdf"], 128.0)
+                g, _ = net.backward(tt["inputs"], dout)
+                g = g.double() / 128.0
+            return o["eval_pdf"].double(), o["raw"].double(), g
+
+        def mixture(raw):
+            K = raw.shape[0] // 4
+            x = raw[:4 * K].reshape(K, 4, -1)
+            th = torch.pi * torch.sigmoid(x[:, 0])
+            ph = 2 * torch.pi * torch.sigmoid(x[:, 1])
+            kap = torch.exp(x[:, 2]).clamp(1e-2, 1e3)
+            w = torch.softmax(x[:, 3], dim=0)
+            mu = torch.stack((torch.sin(th) * torch.cos(ph), torch.sin(th) * torch.sin(ph), torch.cos(th)))
+            return w, kap, mu
+
+        def q(t):
+            t = t.flatten()
+            return {"median": t.median().item(), "p99": t.quantile(0.99).item() if t.numel() <= 1 << 24 else None}
+
+        a, ra, ga = run(ref, True)
+        res = {}
+        for p in precisions:
+            other = npmg.GuidingNetwork(make_cfg(p), seed=7272)
+            other.set_params(params)
+            b, rb, gb = run(other, p != "fp8")
+            rel = ((b - a).abs() / a).flatten()
+            r = {"median": rel.median().item(), "p99": rel.quantile(0.99).item(), "max": rel.max().item(),
+                 "queries": n, "trained_steps": train_steps}
+            if mix_ok:
+                wa, ka, ma = mixture(ra)
+                wb, kb, mb = mixture(rb)
+                ang = torch.acos((ma * mb).sum(0).clamp(-1.0, 1.0))
+                r["mixture"] = {"weight_abs": q((wb - wa).abs()), "kappa_rel": q(((kb - ka) / ka).abs()),
+                                "mean_angle_rad": q(ang)}
+            if gb is not None:
+                nm = other.n_mlp
+                r["gradient_rel_l2"] = {
+                    "mlp": ((gb[:nm] - ga[:nm]).norm() / ga[:nm].norm()).item(),
+                    "grid": ((gb[nm:] - ga[nm:]).norm() / ga[nm:].norm()).item(), "records": n_grad}
+            other.close()
+            res[p] = r
         ref.close()
-        other.close()
-        return {"median": rel.median().item(), "p99": rel.quantile(0.99).item() if rel.numel() <= 1 << 24 else None,
-                "max": rel.max().item(), "queries": n, "trained_steps": train_steps}
+        return res
 
     def asg_cfg(precision):
         c = npmg.benchmark_config("full", precision=precision, width=128, hidden=3)
@@ -365,7 +418,7 @@ def main():
     fprec = "bf16" if args.precision == "bf16x3" else args.precision
     if args.only == "frame":
         frame = {p: measure_frame(p) for p in ("fp32", fprec)}
-        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, fprec, False, train_steps=100)
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, [fprec], False, train_steps=100)[fprec]
         print(json.dumps(frame), flush=True)
         return
     if args.only == "train":
@@ -517,15 +570,14 @@ def main():
         "heldout_loss_after_500_steps": {p: loss_after(p) for p in ("fp32", "bf16", "fp16", "bf16x3")},
     }
     if half:
-        result["pdf_rel_diff_vs_fp32_same_params"] = {
-            p: pdf_agreement(full_cfg, p, True) for p in ("bf16", "fp16", "bf16x3", "fp8")}
+        result["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(full_cfg, ["bf16", "fp16", "bf16x3", "fp8"], True)
         result["fp8_inference_speedup_vs_fp32"] = per_prec["fp32"] / per_prec["fp8"]
         frame = {p: measure_frame(p) for p in ("fp32", fprec)}
         frame["workload"] = ("one frame of 8 bounce queues of floor(2,073,600 * 0.68^b) queries, one persistent "
                              "launch; anisotropic mixture 16 lobes x 6, MLP 3x128, hash grid 16x2^19x2, "
                              "sample + 4 eval pdfs; train step 2^18 samples as one graph")
         frame[f"{fprec}_speedup_vs_fp32"] = frame["fp32"]["ms_per_frame"] / frame[fprec]["ms_per_frame"]
-        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, fprec, False, train_steps=100)
+        frame["pdf_rel_diff_vs_fp32_same_params"] = pdf_agreement(asg_cfg, [fprec], False, train_steps=100)[fprec]
         result["anisotropic_frame"] = frame
     if "fp32" in per_prec and half:
         result[f"{args.precision}_speedup_vs_fp32"] = per_prec["fp32"] / per_prec[args.precision]
