@@ -392,13 +392,20 @@ __device__ __forceinline__ float asg_adjoint(const DevCfg& c, Raw&& x, float th_
     return loss;
 }
 
-// The loss adjoint of either mixture (P raw outputs per lobe).
+// The loss adjoint of either mixture (P raw outputs per lobe). The vMF
+// adjoint takes the math policy of the configuration's numerics (hardware
+// transcendentals with "fast"); the anisotropic one stays exact.
 template <int K, int P, class Raw, class Out>
 __device__ __forceinline__ float lobe_adjoint(const DevCfg& c, Raw&& x, float th_t, float ph_t, float r, float g,
                                               float b, float wipdf, float bsdf_pdf, float ls_over_n, float clamp_hi,
                                               Out&& out) {
-    if constexpr (P == 6) return asg_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, ls_over_n, clamp_hi, out);
-    else return mixture_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, bsdf_pdf, ls_over_n, clamp_hi, out);
+    if constexpr (P == 6) {
+        return asg_adjoint<K>(c, x, th_t, ph_t, r, g, b, wipdf, ls_over_n, clamp_hi, out);
+    } else {
+        if (c.numerics == NUM_FAST)
+            return mixture_adjoint<K, FastMath>(c, x, th_t, ph_t, r, g, b, wipdf, bsdf_pdf, ls_over_n, clamp_hi, out);
+        return mixture_adjoint<K, ExactMath>(c, x, th_t, ph_t, r, g, b, wipdf, bsdf_pdf, ls_over_n, clamp_hi, out);
+    }
 }
 
 }  // namespace dev

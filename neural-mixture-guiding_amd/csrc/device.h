@@ -711,6 +711,7 @@ __device__ __forceinline__ void vmf_query_cfg(const DevCfg& c, Raw&& x, const Qu
 
 // Derivative of 1/k - coth(k): reference polynomial below 1 (distribution.h:
 // 162-168) or the exact value (series below 0.5).
+template <class MT = ExactMath>
 __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
     if (ref) {
         if (k < 1.0f) return 0.000926f + -0.344883f * k + 0.030147f * (k * k);
@@ -720,8 +721,8 @@ __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
         const float k2 = k * k;
         return -k * (1.0f / 3.0f - k2 * (1.0f / 45.0f - k2 * (2.0f / 945.0f - k2 * (1.0f / 4725.0f))));
     }
-    const float e = fm::exp(-2.0f * k);
-    return 1.0f / k - (1.0f + e) / (1.0f - e);
+    const float e = MT::exp(-2.0f * k);
+    return MT::div(1.0f, k) - MT::div(1.0f + e, 1.0f - e);
 }
 
 // KL loss -f/q log(p + eps) and its adjoint with respect to the raw outputs
@@ -735,13 +736,15 @@ __device__ __forceinline__ float inv_k_minus_coth(float k, bool ref) {
 // -f/P log P, dL/dx = -f/P^2 dP/dx with dP/dlobe = s dp/dlobe and the
 // selection slot 0.01 (p - bsdf_pdf) s (1 - s); no NaN zeroing or clamp.
 //
+// MT: the math policy of the configuration's numerics (FastMath with "fast",
+// whose formulas are the stable ones on hardware transcendentals).
 // c.half_adjoint reproduces the reference's FP16 intermediates: the raw
 // outputs are read as the network's __half outputs (train.cu:80-81), the four
 // per-lobe dp/d(param) are stored in __half by gradients_probability
 // (distribution.h:143-181, its output is precision_t), and each dL/dx is
 // stored in __half before the NaN test and the clamp (train.cu:95-123), so a
 // value beyond the FP16 range becomes infinite and is zeroed, not clamped.
-template <int K, class Raw0, class Out>
+template <int K, class MT = ExactMath, class Raw0, class Out>
 __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw0&& x0, float th_t, float ph_t, float r, float g,
                                                  float b, float wipdf, float bsdf_pdf, float ls_over_n,
                                                  float clamp_hi, Out&& out) {
@@ -750,29 +753,29 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw0&& x0, flo
     auto x = [&](int j) { return h16 ? fm::round_half(x0(j)) : x0(j); };
     auto hq = [&](float v) { return h16 ? fm::round_half(v) : v; };
     Mixture<K> m;
-    build_mixture<K>(c, x, m);
-    const V3 wi = sph_to_cart(th_t, ph_t);
+    build_mixture<K, MT>(c, x, m);
+    const V3 wi = sph_to_cart<MT>(th_t, ph_t);
     const float Li = (r + g + b) / 3.0f;
     const float wiPdf = wipdf + kEps;
     float vmf[K];
     float p = 0.0f;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-        vmf[s] = lobe_pdf(c, m, s, wi);
+        vmf[s] = lobe_pdf<K, MT>(c, m, s, wi);
         p = fm::fma_(m.weight[s], vmf[s], p);
     }
     const float guide = p + kEps;
     float prefix, loss, sel_s = 1.0f, sel_grad = 0.0f;
     if (c.sel) {
-        const float s = 1.0f / (1.0f + fm::exp(-x(4 * K)));
+        const float s = MT::div(1.0f, 1.0f + MT::exp(-x(4 * K)));
         const float P = s * guide + (1.0f - s) * bsdf_pdf;
         prefix = -Li / P / P * ls_over_n;
-        loss = -Li / P * fm::log(P);
+        loss = -Li / P * MT::log(P);
         sel_s = s;
         sel_grad = hq(0.01f * prefix * hq(p - bsdf_pdf) * (s * (1.0f - s)));
     } else {
         prefix = -Li / wiPdf / guide * ls_over_n;
-        loss = -Li / wiPdf * fm::log(guide);
+        loss = -Li / wiPdf * MT::log(guide);
     }
     // dp/d(param) of a lobe, times s with a learned selection (stored again:
     // distribution.h:224-226)
@@ -791,13 +794,13 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw0&& x0, flo
         const V3 mu = m.mu[s];
         // mu = (sin t cos p, sin t sin p, cos t): recover the derivative
         // directions from mu and sin t (no second trigonometric evaluation).
-        const float st = fm::sqrt_(smax(0.0f, mu.x * mu.x + mu.y * mu.y));
+        const float st = MT::sqrt(smax(0.0f, mu.x * mu.x + mu.y * mu.y));
         const float wv = m.weight[s] * vmf[s];
-        const float dl = hq((vmf[s] - p) / m.total);
-        const float dk = hq(wv * (dot(wi, mu) + inv_k_minus_coth(kap, ref)));
+        const float dl = hq(MT::div(vmf[s] - p, m.total));
+        const float dk = hq(wv * (dot(wi, mu) + inv_k_minus_coth<MT>(kap, ref)));
         // d mu / d theta = (cos t cos p, cos t sin p, -sin t); d mu / d phi = sin t (-sin p, cos p, 0).
         float cp = 1.0f, sp = 0.0f;
-        if (st > 0.0f) { cp = mu.x / st; sp = mu.y / st; }
+        if (st > 0.0f) { cp = MT::div(mu.x, st); sp = MT::div(mu.y, st); }
         const float dth = hq(wv * kap * (mu.z * (wi.x * cp + wi.y * sp) - wi.z * st));
         const float dph = hq(wv * kap * (wi.y * cp - wi.x * sp) * st);
         if (c.mapping == MAP_REFERENCE) {
@@ -814,9 +817,9 @@ __device__ __forceinline__ float mixture_adjoint(const DevCfg& c, Raw0&& x0, flo
             finish(4 * s + 3, prefix * sc(dph) * kTwoPi * (s3 * (1.0f - s3)));
         } else {
             const float x0 = x(4 * s + 0), x1 = x(4 * s + 1), x2 = x(4 * s + 2);
-            const float s0 = 1.0f / (1.0f + fm::exp(-x0));
-            const float s1 = 1.0f / (1.0f + fm::exp(-x1));
-            const float ek = fm::exp(x2);
+            const float s0 = MT::div(1.0f, 1.0f + MT::exp(-x0));
+            const float s1 = MT::div(1.0f, 1.0f + MT::exp(-x1));
+            const float ek = MT::exp(x2);
             const float dkx = (ek > 1e-2f && ek < 1e3f) ? ek : 0.0f;
             finish(4 * s + 0, prefix * sc(dth) * kPi * (s0 * (1.0f - s0)));
             finish(4 * s + 1, prefix * sc(dph) * kTwoPi * (s1 * (1.0f - s1)));

@@ -30,10 +30,16 @@ def _configs():
         "refnum_h16": dict(npmg.benchmark_config("small"),
                            mixture={"otype": "vMF", "n_lobes": 8, "mapping": "reference", "numerics": "reference"},
                            loss={"scale": 128.0, "grad_clamp": 150.0, "fp16_intermediates": True}),
+        # the fast numerics (hardware transcendentals): not bit-exact, so only
+        # in the tests with a tolerance
+        "small_fast": dict(npmg.benchmark_config("small"),
+                           mixture={"otype": "vMF", "n_lobes": 8, "mapping": "softmax", "numerics": "fast"}),
+        "k16_fast": dict(npmg.benchmark_config("small", n_lobes=16),
+                         mixture={"otype": "vMF", "n_lobes": 16, "mapping": "softmax", "numerics": "fast"}),
     }
 
 
-CONFIGS = list(_configs().keys())
+CONFIGS = [k for k in _configs() if not k.endswith("_fast")]
 
 
 @pytest.fixture(scope="module")
@@ -172,7 +178,7 @@ def test_mixture_consumer_bitwise(nets, oracle, name, cosine):
         assert np.array_equal(_np(out[k]).view(np.uint32), ref[k].view(np.uint32)), k
 
 
-@pytest.mark.parametrize("name", ["small", "small_refnum", "k16", "refnum_h16"])
+@pytest.mark.parametrize("name", ["small", "small_refnum", "k16", "refnum_h16", "small_fast", "k16_fast"])
 def test_adjoint_vs_fp64(nets, oracle, name):
     import torch
     import npmg
