@@ -79,7 +79,10 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
     const int n_to = ow / 32, n_ti = (iw + 31) / 32;
     // groups: (input tile, pair of output tiles), or single output tiles when
     // pairs would leave waves without work
-    const int tpg = n_ti * ((n_to + 1) / 2) >= 4 ? 2 : 1;
+    // (split operands: one output tile per group, so that a wave's operands
+    // of a whole block fit at two waves per SIMD)
+    constexpr int TPG = NP == 2 ? 1 : 2;
+    const int tpg = TPG == 2 && n_ti * ((n_to + 1) / 2) >= 4 ? 2 : 1;
     const int n_grp = n_ti * ((n_to + tpg - 1) / tpg);
     float* P = partials + (size_t)row * n_mlp + c.layer_off[l];
     // 16-byte fragment loads need 16-byte aligned rows: n a multiple of 8.
@@ -140,26 +143,92 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
         const int per_ti = (n_to + tpg - 1) / tpg;
         const int ti = grp / per_ti, to0 = (grp % per_ti) * tpg;
         const int nto = n_to - to0 < tpg ? n_to - to0 : tpg;
-        mf::f32x16 acc[2] = {mf::f32x16{}, mf::f32x16{}};
-        const E* ga[2];
+        mf::f32x16 acc[TPG];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) ga[a] = G + (size_t)(32 * (to0 + (a < nto ? a : 0)) + j) * n;
+        for (int a = 0; a < TPG; ++a) acc[a] = mf::f32x16{};
+        const E* ga[TPG];
+#pragma unroll
+        for (int a = 0; a < TPG; ++a) ga[a] = G + (size_t)(32 * (to0 + (a < nto ? a : 0)) + j) * n;
         // rows beyond iw (a partial tile) read row iw - 1 and are not stored
         const int xi = 32 * ti + j < iw ? 32 * ti + j : iw - 1;
         const E* xb = X + (size_t)xi * n;
+        // Where the group's input rows come from, the same for the whole wave:
+        // the stored operands (0), or for rebuilt layer-0 inputs a tile of grid
+        // features (1), of OneBlob bins (2) or of both (3, lane by lane). Each
+        // case issues all of a block's loads before any use (a per-lane choice
+        // between them made the compiler wait for each load in turn).
+        const int kind = __builtin_amdgcn_readfirstlane(
+            !rebuild ? 0 : ((32 * ti + 31) >> 4) < c.ob_kb0 ? 1 : ((32 * ti) >> 4) >= c.ob_kb0 ? 2 : 3);
+        auto load_a = [&](uint32_t q, bool full, V (&av)[TPG][4]) {
+#pragma unroll
+            for (int a = 0; a < TPG; ++a)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) av[a][s] = load(ga[a], pg, q, s, full);
+        };
         auto block = [&](uint32_t q, bool full) {
-            V av[2][4], bv[4];
+            V av[TPG][4], bv[4];
+            if (kind == 0) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) bv[s] = rebuild ? load_x0(xi, q, s, full) : load(xb, px, q, s, full);
+                for (int s = 0; s < 4; ++s) bv[s] = load(xb, px, q, s, full);
+                load_a(q, full, av);
+            } else if (kind == 1) {
+                // grid features of row xi (16-bit), converted after all loads are out
+                GT g[4][8];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+                for (int s = 0; s < 4; ++s) {
+                    const uint32_t q0 = q + 32u * (uint32_t)h + 8u * (uint32_t)s;
+                    const GT* src = genc + (size_t)xi * n + q0;
+                    if (full) {
+                        const uint4 raw = *reinterpret_cast<const uint4*>(src);
+                        __builtin_memcpy(g[s], &raw, 16);
+                    } else {
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    if (a < nto) av[a][s] = load(ga[a], pg, q, s, full);
+                        for (int e = 0; e < 8; ++e) g[s][e] = q0 + (uint32_t)e < qe ? src[e] : (GT)0.0f;
+                    }
+                }
+                load_a(q, full, av);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    float v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = (float)g[s][e];
+                    bv[s] = mf::pack8<T>(v);
+                }
+            } else if (kind == 2) {
+                // OneBlob input of row xi's dimension (FP32), binned after all loads are out
+                const float* row = in + (size_t)(c.ob_in + (xi >> 4) - c.ob_kb0) * n;
+                float x[4][8];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const uint32_t q0 = q + 32u * (uint32_t)h + 8u * (uint32_t)s;
+                    if (full) {
+                        const float4 a = *reinterpret_cast<const float4*>(row + q0);
+                        const float4 b = *reinterpret_cast<const float4*>(row + q0 + 4);
+                        x[s][0] = a.x; x[s][1] = a.y; x[s][2] = a.z; x[s][3] = a.w;
+                        x[s][4] = b.x; x[s][5] = b.y; x[s][6] = b.z; x[s][7] = b.w;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) x[s][e] = q0 + (uint32_t)e < qe ? row[q0 + e] : 0.0f;
+                    }
+                }
+                load_a(q, full, av);
+                const int bin = xi & 15;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    float v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = oneblob16_bin(x[s][e], bin);
+                    bv[s] = mf::pack8<T>(v);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) bv[s] = load_x0(xi, q, s, full);
+                load_a(q, full, av);
+            }
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < TPG; ++a)
                     if (a < nto) acc[a] = mf::Frag<T>::mma(av[a][s], bv[s], acc[a]);
         };
         uint32_t q = qb;
@@ -169,7 +238,7 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
         const int i = 32 * ti + j;
         if (i < iw) {
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < TPG; ++a) {
                 if (a >= nto) break;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
