@@ -35,9 +35,9 @@ __device__ __forceinline__ float oneblob16_bin(float x, int bin) {
 }
 
 // Weight gradients on MFMA: dW_l[o][i] = sum_q dY_l[o][q] X_l[i][q] with the
-// batch as the K dimension. Block = (layer, one range of q_per_row samples, a
-// multiple of 64), whose partial row of dW_l a deterministic reduction over the
-// ranges sums afterwards. The layer's 32 x 32 output tiles are cut into
+// batch as the K dimension. One launch per layer; block = one range of
+// q_per_row samples (a multiple of 64), whose partial row of dW_l a
+// deterministic reduction over the ranges sums afterwards. The layer's 32 x 32 output tiles are cut into
 // groups of one input tile and up to two output tiles; the 4 waves of the
 // block take the groups in turn over the same samples, so an operand row
 // loaded by several waves is fetched once and served to the others by the
@@ -58,12 +58,12 @@ template <class T>
 __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainMfmaWS ws,
                                                        const typename mf::Op<T>::table* __restrict__ genc,
                                                        const float* __restrict__ in, float* __restrict__ partials,
-                                                       uint32_t n, uint32_t q_per_row, uint32_t n_mlp) {
+                                                       uint32_t n, uint32_t q_per_row, uint32_t n_mlp, int layer) {
     using V = typename mf::Frag<T>::vec;
     using E = typename mf::Op<T>::elem;
     using GT = typename mf::Op<T>::table;
     constexpr int NP = mf::Op<T>::planes;
-    const int l = blockIdx.x;
+    const int l = layer;
     const int H = c.n_layers - 1;
     const int iw = c.layer_in[l], ow = c.layer_out[l];
     const E* X = l == 0 ? static_cast<const E*>(ws.x0) : static_cast<const E*>(ws.xh) + (size_t)(l - 1) * c.width * n;
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) k_wgrad_mfma(const DevCfg c, const TrainM
     const size_t pg = (l == H ? (size_t)c.n_out_pad : (size_t)H * c.width) * n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, j = lane & 31;
-    const uint32_t row = blockIdx.y;
+    const uint32_t row = blockIdx.x;
     const uint32_t qb = row * q_per_row;
     const uint32_t qe = min(n, qb + q_per_row);
     const int n_to = ow / 32, n_ti = (iw + 31) / 32;
@@ -278,17 +278,22 @@ hipError_t launch_wgrad_mfma(const DevCfg& c, int precision, const TrainMfmaWS& 
     uint32_t rows, qpw;
     wgrad_split(n, &rows, &qpw);
     if ((size_t)rows * n_mlp > partials_cap) return hipErrorInvalidValue;
-    const dim3 grid(c.n_layers, rows);
-    if (precision == PREC_BF16)
-        hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, static_cast<const __bf16*>(genc),
-                           in, partials, n, qpw, n_mlp);
-    else if (precision == PREC_FP16)
-        hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws, static_cast<const _Float16*>(genc),
-                           in, partials, n, qpw, n_mlp);
-    else if (precision == PREC_BF16X3)
-        hipLaunchKernelGGL((k_wgrad_mfma<mf::Bf16x3>), grid, dim3(256), 0, s, c, ws,
-                           static_cast<const _Float16*>(genc), in, partials, n, qpw, n_mlp);
-    else return hipErrorInvalidValue;
+    // one launch per layer: each streams one layer's operands (a single launch
+    // over all layers interleaved their workgroups and took 0.27 ms against
+    // 0.22 ms for the four launches one after another, split BF16 at 2^18)
+    const dim3 grid(rows);
+    for (int l = 0; l < c.n_layers; ++l) {
+        if (precision == PREC_BF16)
+            hipLaunchKernelGGL((k_wgrad_mfma<__bf16>), grid, dim3(256), 0, s, c, ws, static_cast<const __bf16*>(genc),
+                               in, partials, n, qpw, n_mlp, l);
+        else if (precision == PREC_FP16)
+            hipLaunchKernelGGL((k_wgrad_mfma<_Float16>), grid, dim3(256), 0, s, c, ws,
+                               static_cast<const _Float16*>(genc), in, partials, n, qpw, n_mlp, l);
+        else if (precision == PREC_BF16X3)
+            hipLaunchKernelGGL((k_wgrad_mfma<mf::Bf16x3>), grid, dim3(256), 0, s, c, ws,
+                               static_cast<const _Float16*>(genc), in, partials, n, qpw, n_mlp, l);
+        else return hipErrorInvalidValue;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_reduce_splits(partials, rows, n_mlp, dw, s, overflow);
