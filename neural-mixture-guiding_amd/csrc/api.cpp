@@ -39,7 +39,9 @@ hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void*
 hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
                                     int n_cu, hipStream_t s);
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
-                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve);
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve,
+                                    int n_lds, int levels);
+int grid_encode_lds_levels(const DevCfg& c, const void* table, uint32_t batch);
 bool mfma_split_grid(const DevCfg& c);
 hipError_t launch_pack_frags(const DevCfg& c, int precision, const float* w, void* frag, hipStream_t s);
 hipError_t launch_to_half(int precision, const float* src, void* dst, size_t n, hipStream_t s);
@@ -413,9 +415,10 @@ struct npmg_ctx {
             // kernel on s after chunk c's encode.
             const uint32_t chunks = n >= (1u << 18) ? tune_chunks : 1u;
             const uint32_t m = ((n + chunks - 1u) / chunks + 63u) / 64u * 64u;
+            const int n_lds = grid_encode_lds_levels(cfg.d, tab, n);
             if (chunks == 1u) {
                 run(NPMG_TIME_GRID_ENCODE, s, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, 0u, n, s, 0u),
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, 0u, n, s, 0u, n_lds, ENC_ALL),
                               "grid encode");
                 });
                 run(NPMG_TIME_QUERY, s, [&] {
@@ -426,10 +429,18 @@ struct npmg_ctx {
             }
             hip_check(hipEventRecord(ev_fork, s), "event");
             hip_check(hipStreamWaitEvent(aux, ev_fork, 0), "stream wait");
+            // the LDS-staged coarse levels of the whole batch first, then the gathered levels chunk by chunk
+            if (n_lds > 0)
+                run(NPMG_TIME_GRID_ENCODE, aux, [&] {
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, 0u, n, aux, 0u, n_lds,
+                                                       ENC_LDS_LEVELS),
+                              "grid encode");
+                });
             for (uint32_t k = 0; k < chunks; ++k) {
                 const uint32_t b = std::min(n, k * m), e = std::min(n, b + m);
                 run(NPMG_TIME_GRID_ENCODE, aux, [&] {
-                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, b, e, aux, kEncodeLdsReserve),
+                    hip_check(launch_grid_encode_range(cfg.d, cfg.precision, tab, io.in, g, n, b, e, aux,
+                                                       kEncodeLdsReserve, n_lds, ENC_GATHER_LEVELS),
                               "grid encode");
                 });
                 hip_check(hipEventRecord(ev_chunk[k], aux), "event");

@@ -14,12 +14,15 @@ enum HashType : int { HASH_COHERENT_PRIME = 0, HASH_PRIME = 1, HASH_REVERSED_PRI
 enum Interp : int { INTERP_LINEAR = 0, INTERP_SMOOTHSTEP = 1, INTERP_NEAREST = 2 };
 enum Mapping : int { MAP_REFERENCE = 0, MAP_SOFTMAX = 1 };
 // fast: the stable formulas with hardware transcendentals on the query path
-// (the training adjoint keeps the exact functions).
+// and in the vMF training adjoint (the ASG adjoint keeps the exact functions).
 enum Numerics : int { NUM_REFERENCE = 0, NUM_STABLE = 1, NUM_FAST = 2 };
 enum MixtureType : int { MIX_VMF = 0, MIX_ASG = 1 };
 // PREC_BF16X3: MFMA on split BF16 operands (hi + lo, three products) with the
 // grid table and level-major features in FP16.
 enum Precision : int { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2, PREC_FP8 = 3, PREC_BF16X3 = 4 };
+// Which grid levels a level-major encode launch covers: all, only the coarse
+// levels whose tables are staged in LDS, or only the gathered ones.
+enum EncodeLevels : int { ENC_ALL = 0, ENC_LDS_LEVELS = 1, ENC_GATHER_LEVELS = 2 };
 
 constexpr int kMaxEnc = 4;
 constexpr int kMaxLevels = 32;

@@ -7,6 +7,8 @@
 // when the grid rows form whole 8-feature chunks; the host pipelines it with the
 // fused kernel over chunks of the batch on two streams, since the encode is
 // bound by L2 gather bandwidth and the fused kernel by VALU issue.
+#include <algorithm>
+
 #include "asg.h"
 #include "device.h"
 #include "kernels.h"
@@ -32,12 +34,14 @@ using namespace dev;
 // linear order, so the whole GPU works on one level at a time and that
 // level's table slice (at most 2^19 x F entries) stays resident in each XCD's
 // L2 while it is gathered; the features go out in the operand type, [L F][n].
+// l0: the first level of the launch (the levels below it are gathered from LDS,
+// k_grid_encode_lds).
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __restrict__ table,
                                                      const float* __restrict__ in, T* __restrict__ genc, uint32_t n,
-                                                     uint32_t q_begin, uint32_t q_end) {
+                                                     uint32_t q_begin, uint32_t q_end, int l0) {
     const uint32_t q = q_begin + blockIdx.x * 256u + threadIdx.x;
-    const int l = blockIdx.y;
+    const int l = blockIdx.y + l0;
     if (q >= q_end) return;
     const int b = c.enc_begin[c.grid_enc];
     const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
@@ -63,6 +67,57 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
                 for (int k = 0; k < 8; ++k) a = fm::fma_(w[k], (float)tl[idx[k] * (uint32_t)c.F + ff], a);
             genc[(size_t)(c.F * l + ff) * n + q] = (T)a;
         }
+    }
+}
+
+// The coarse levels whose table fits in LDS (F = 2: one 32-bit word per
+// entry, at most kLdsLevelBytes): grid (workgroups, levels); a workgroup of
+// 1024 threads stages its level's table in LDS once and gathers the corners
+// of many queries from there (grid-stride over the queries), instead of four
+// L2 requests per query and level; two such workgroups fill a CU. The
+// interpolation is grid_level_features' (same operations, same order), so the
+// features are the same bits.
+constexpr uint32_t kLdsLevelBytes = 64u * 1024u;
+constexpr uint32_t kLdsEncodeThreads = 1024u;
+template <class T>
+__global__ void __launch_bounds__(1024) k_grid_encode_lds(const DevCfg c, const T* __restrict__ table,
+                                                          const float* __restrict__ in, T* __restrict__ genc,
+                                                          uint32_t n, uint32_t q_begin, uint32_t q_end) {
+    static_assert(sizeof(T) == 2, "16-bit features");
+    extern __shared__ uint32_t lt[];
+    const int l = blockIdx.y;
+    const uint32_t entries = c.offsets[l + 1] - c.offsets[l];
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(table + (size_t)c.offsets[l] * 2u);
+    for (uint32_t i = threadIdx.x; i < entries; i += kLdsEncodeThreads) lt[i] = src[i];
+    __syncthreads();
+    const int b = c.enc_begin[c.grid_enc];
+    auto entry = [&](uint32_t i, float& f0, float& f1) {
+        const uint32_t wv = lt[i];
+        T v[2];
+        __builtin_memcpy(v, &wv, 4);
+        f0 = (float)v[0];
+        f1 = (float)v[1];
+    };
+    for (uint32_t q = q_begin + blockIdx.x * kLdsEncodeThreads + threadIdx.x; q < q_end;
+         q += gridDim.x * kLdsEncodeThreads) {
+        const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
+        uint32_t idx[8];
+        float w[8];
+        const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+        float a0 = 0.0f, a1 = 0.0f;
+        if (nc == 1) {
+            entry(idx[0], a0, a1);
+        } else {
+            float v0[8], v1[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) entry(idx[k], v0[k], v1[k]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a0 = fm::fma_(w[k], v0[k], a0);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a1 = fm::fma_(w[k], v1[k], a1);
+        }
+        genc[(size_t)(2 * l) * n + q] = (T)a0;
+        genc[(size_t)(2 * l + 1) * n + q] = (T)a1;
     }
 }
 
@@ -285,28 +340,70 @@ hipError_t launch_mfma_type(const DevCfg& c, const void* wfrag, const void* tabl
 
 }  // namespace
 
+// The number of leading levels a batch of `batch` queries gathers from LDS:
+// those whose F = 2 tables fit in kLdsLevelBytes, when the batch gives each
+// of 256 workgroups per level at least 4096 queries to repay the staging
+// (82 KB per CU for the full configuration's levels 0 and 1; at 2^18 queries
+// the staging costs what it saves).
+constexpr uint32_t kLdsMinBatch = 1u << 20;
+int grid_encode_lds_levels(const DevCfg& c, const void* table, uint32_t batch) {
+    int n_lds = 0;
+    if (batch < kLdsMinBatch || c.F != 2 || (reinterpret_cast<uintptr_t>(table) & 3u) != 0u) return 0;
+    while (n_lds < c.n_levels && (c.offsets[n_lds + 1] - c.offsets[n_lds]) * 4u <= kLdsLevelBytes) ++n_lds;
+    return n_lds;
+}
+
 // lds_reserve: unused dynamic LDS per workgroup, which caps the encode's
 // workgroups per CU so that fused-kernel workgroups running beside it (the
 // pipelined query path) still find room; the encode loses nothing down to
-// about 5 workgroups per CU.
+// about 5 workgroups per CU. n_lds: the leading levels gathered from LDS
+// (grid_encode_lds_levels). levels: ENC_ALL, or only those n_lds levels
+// (ENC_LDS_LEVELS) or only the others (ENC_GATHER_LEVELS), so that a
+// pipelined caller can encode the coarse levels of the whole batch once (a
+// 1024-thread, 64 KB workgroup does not fit beside the fused kernel).
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
-                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve) {
+                                    uint32_t n, uint32_t q_begin, uint32_t q_end, hipStream_t s, uint32_t lds_reserve,
+                                    int n_lds, int levels) {
     if (q_end <= q_begin) return hipSuccess;
-    const dim3 grid((q_end - q_begin + 255) / 256, c.n_levels);
-    if (precision == PREC_BF16)
-        hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), lds_reserve, s, c, (const __bf16*)table, in,
-                           (__bf16*)genc, n, q_begin, q_end);
-    else if (precision == PREC_FP16 || precision == PREC_BF16X3)
-        hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), lds_reserve, s, c, (const _Float16*)table, in,
-                           (_Float16*)genc, n, q_begin, q_end);
-    else
+    if (precision != PREC_BF16 && precision != PREC_FP16 && precision != PREC_BF16X3) return hipErrorInvalidValue;
+    if (n_lds < 0 || n_lds > c.n_levels || (n_lds > 0 && (c.F != 2 || (reinterpret_cast<uintptr_t>(table) & 3u))))
         return hipErrorInvalidValue;
+    const bool bf = precision == PREC_BF16;
+    uint32_t lds_bytes = 0;
+    for (int l = 0; l < n_lds; ++l) {
+        const uint32_t b = (c.offsets[l + 1] - c.offsets[l]) * 4u;
+        if (b > kLdsLevelBytes) return hipErrorInvalidValue;
+        lds_bytes = std::max(lds_bytes, b);
+    }
+    const uint32_t tiles = (q_end - q_begin + 255) / 256;
+    if (n_lds > 0 && levels != ENC_GATHER_LEVELS) {
+        // one workgroup per CU and level at most, each over >= 1 tile of kLdsEncodeThreads queries
+        const uint32_t wgs = std::min((q_end - q_begin + kLdsEncodeThreads - 1) / kLdsEncodeThreads, 256u);
+        const dim3 grid_l(wgs, (uint32_t)n_lds);
+        const uint32_t lds = std::max(lds_bytes, lds_reserve);
+        if (bf)
+            hipLaunchKernelGGL((k_grid_encode_lds<__bf16>), grid_l, dim3(kLdsEncodeThreads), lds, s, c,
+                               (const __bf16*)table, in, (__bf16*)genc, n, q_begin, q_end);
+        else
+            hipLaunchKernelGGL((k_grid_encode_lds<_Float16>), grid_l, dim3(kLdsEncodeThreads), lds, s, c,
+                               (const _Float16*)table, in, (_Float16*)genc, n, q_begin, q_end);
+    }
+    if (n_lds < c.n_levels && levels != ENC_LDS_LEVELS) {
+        const dim3 grid(tiles, (uint32_t)(c.n_levels - n_lds));
+        if (bf)
+            hipLaunchKernelGGL((k_grid_encode<__bf16>), grid, dim3(256), lds_reserve, s, c, (const __bf16*)table, in,
+                               (__bf16*)genc, n, q_begin, q_end, n_lds);
+        else
+            hipLaunchKernelGGL((k_grid_encode<_Float16>), grid, dim3(256), lds_reserve, s, c, (const _Float16*)table,
+                               in, (_Float16*)genc, n, q_begin, q_end, n_lds);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_grid_encode(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
                               uint32_t n, hipStream_t s) {
-    return launch_grid_encode_range(c, precision, table, in, genc, n, 0u, n, s, 0u);
+    return launch_grid_encode_range(c, precision, table, in, genc, n, 0u, n, s, 0u,
+                                    grid_encode_lds_levels(c, table, n), ENC_ALL);
 }
 
 bool mfma_split_grid(const DevCfg& c) {

@@ -143,6 +143,26 @@ def test_queue_limits(gpu):
     net.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
+def test_lds_levels_match_gathered(gpu, prec):
+    """Batches of 2^20 queries or more gather the coarse levels whose tables
+    fit in LDS (levels 0 and 1 of the full configuration) from a copy staged
+    there, pipelined or in one launch; a queue launch gathers every level from
+    the table in memory. The features, and so every output, are the same bits."""
+    import torch
+
+    net = _net(prec, "vMF", "full")
+    n = (1 << 20) + 4099  # a ragged tail past whole 1024-query workgroups
+    (q,) = _queues([n], [n], 77, True)
+    cnt = torch.tensor([n], dtype=torch.int32, device="cuda")
+    (ref,) = net.sample_pdf_queues([q], cnt, n_eval=4)
+    torch.cuda.synchronize()
+    for shape in ((1, 0), (4, 0), (3, 2)):
+        net.set_tuning(*shape)
+        _check_queue(net, q, ref, n, n, True, 4)
+    net.close()
+
+
 @pytest.mark.parametrize("prec,mix", [("bf16", "vMF"), ("bf16x3", "vMF"), ("bf16", "ASG")])
 def test_autotune_keeps_results(gpu, prec, mix):
     import torch
