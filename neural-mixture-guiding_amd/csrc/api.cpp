@@ -363,8 +363,10 @@ struct npmg_ctx {
     // Kernel layouts of the parameters: transposed FP32 weights, MFMA fragments and,
     // unless the optimiser already wrote them, the operand-type grid tables.
     void refresh_layouts(hipStream_t s, bool tables = true) {
-        hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
-        hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
+        // The transposed FP32 weights feed the FP32 kernels only: in the 16-bit
+        // precisions, just the FP8 calibration's FP32 forward (inference weights).
+        if (!half()) hip_check(launch_transpose_mlp(cfg.d, master, wt_train, s), "transpose");
+        if (!half() || cfg.infer_fp8) hip_check(launch_transpose_mlp(cfg.d, infer_params(), wt_infer, s), "transpose");
         if (half()) {
             hip_check(launch_pack_frags(cfg.d, cfg.precision, master, frag_train, s), "pack");
             if (bfrag_train) hip_check(launch_pack_bfrags(cfg.d, cfg.precision, master, bfrag_train, s), "pack");
