@@ -190,7 +190,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the guiding path would shard queries by index, see DESIGN.md)", file=sys.stderr)
+              "(the guiding path would shard a frame's queries by index, one parameter replica per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
