@@ -491,7 +491,9 @@ def main():
         encode_roofline = {"bound": "hbm", "achieved": e_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": e_ach / HBM_PEAK_GBS,
                            "traffic": profiled(args.precision, "traffic_bytes", qe, "k_grid_encode"),
-                           "kernel": "k_grid_encode (level-major hash-grid gathers, 16 levels x 8 corners)",
+                           "kernel": "k_grid_encode_lds + k_grid_encode (level-major hash-grid gathers, 16 levels "
+                                     "x 8 corners; levels 0-1 from LDS-staged tables at 2^20 queries and more, "
+                                     "timed as launches of the same tag)",
                            "algorithmic_bytes_per_launch": e_bytes, "kernel_io_bytes_per_launch": e_io,
                            "avg_launch_ms": ke["avg_ms_per_launch"], "launches_per_step": ke["launches_per_step"],
                            "limiter": "L1 cache-line lookups of random gathers (about 5 per query and level), "
