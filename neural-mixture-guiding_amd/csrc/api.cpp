@@ -35,7 +35,7 @@ hipError_t launch_query_mfma_queues(const DevCfg& c, int precision, const void* 
                                     const QueueSet& qs, int n_cu, int blocks_per_cu, const float* fp8_scale,
                                     hipStream_t s);
 hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void* table, const QueueSet& qs, int n_cu,
-                                     hipStream_t s);
+                                     uint64_t capacity, hipStream_t s);
 hipError_t launch_query_fp32_queues(const DevCfg& c, const float* wt, const float* table, const QueueSet& qs,
                                     int n_cu, hipStream_t s);
 hipError_t launch_grid_encode_range(const DevCfg& c, int precision, const void* table, const float* in, void* genc,
@@ -509,7 +509,7 @@ struct npmg_ctx {
                     }
                     qs.counter = queue_counter;
                     qs.batch = tune_items;
-                    hip_check(launch_grid_encode_queues(cfg.d, cfg.precision, htab_infer, qs, n_cu, s),
+                    hip_check(launch_grid_encode_queues(cfg.d, cfg.precision, htab_infer, qs, n_cu, total_cap, s),
                               "grid encode (queues)");
                 }
                 qs.counter = queue_counter + 1;

@@ -79,18 +79,22 @@ __global__ void __launch_bounds__(256) k_grid_encode(const DevCfg c, const T* __
 // features are the same bits.
 constexpr uint32_t kLdsLevelBytes = 64u * 1024u;
 constexpr uint32_t kLdsEncodeThreads = 1024u;
+
+// Level l's table (F = 2 16-bit features: one word per entry) into LDS.
 template <class T>
-__global__ void __launch_bounds__(1024) k_grid_encode_lds(const DevCfg c, const T* __restrict__ table,
-                                                          const float* __restrict__ in, T* __restrict__ genc,
-                                                          uint32_t n, uint32_t q_begin, uint32_t q_end) {
+__device__ __forceinline__ void lds_stage_level(const DevCfg& c, const T* __restrict__ table, int l, uint32_t* lt) {
     static_assert(sizeof(T) == 2, "16-bit features");
-    extern __shared__ uint32_t lt[];
-    const int l = blockIdx.y;
     const uint32_t entries = c.offsets[l + 1] - c.offsets[l];
     const uint32_t* src = reinterpret_cast<const uint32_t*>(table + (size_t)c.offsets[l] * 2u);
     for (uint32_t i = threadIdx.x; i < entries; i += kLdsEncodeThreads) lt[i] = src[i];
     __syncthreads();
-    const int b = c.enc_begin[c.grid_enc];
+}
+
+// Features of query q at level l from the staged table: grid_level_features'
+// corners, weights and FMA order.
+template <class T>
+__device__ __forceinline__ void lds_level_query(const DevCfg& c, const uint32_t* lt, int l, const float* __restrict__ in,
+                                                int b, uint32_t n, uint32_t q, T* __restrict__ genc) {
     auto entry = [&](uint32_t i, float& f0, float& f1) {
         const uint32_t wv = lt[i];
         T v[2];
@@ -98,26 +102,52 @@ __global__ void __launch_bounds__(1024) k_grid_encode_lds(const DevCfg c, const 
         f0 = (float)v[0];
         f1 = (float)v[1];
     };
+    const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
+    uint32_t idx[8];
+    float w[8];
+    const int nc = level_corners(c, l, x0, x1, x2, idx, w);
+    float a0 = 0.0f, a1 = 0.0f;
+    if (nc == 1) {
+        entry(idx[0], a0, a1);
+    } else {
+        float v0[8], v1[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) entry(idx[k], v0[k], v1[k]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a0 = fm::fma_(w[k], v0[k], a0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a1 = fm::fma_(w[k], v1[k], a1);
+    }
+    genc[(size_t)(2 * l) * n + q] = (T)a0;
+    genc[(size_t)(2 * l + 1) * n + q] = (T)a1;
+}
+template <class T>
+__global__ void __launch_bounds__(1024) k_grid_encode_lds(const DevCfg c, const T* __restrict__ table,
+                                                          const float* __restrict__ in, T* __restrict__ genc,
+                                                          uint32_t n, uint32_t q_begin, uint32_t q_end) {
+    extern __shared__ uint32_t lt[];
+    const int l = blockIdx.y;
+    lds_stage_level(c, table, l, lt);
+    const int b = c.enc_begin[c.grid_enc];
     for (uint32_t q = q_begin + blockIdx.x * kLdsEncodeThreads + threadIdx.x; q < q_end;
-         q += gridDim.x * kLdsEncodeThreads) {
-        const float x0 = in[(size_t)b * n + q], x1 = in[(size_t)(b + 1) * n + q], x2 = in[(size_t)(b + 2) * n + q];
-        uint32_t idx[8];
-        float w[8];
-        const int nc = level_corners(c, l, x0, x1, x2, idx, w);
-        float a0 = 0.0f, a1 = 0.0f;
-        if (nc == 1) {
-            entry(idx[0], a0, a1);
-        } else {
-            float v0[8], v1[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) entry(idx[k], v0[k], v1[k]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) a0 = fm::fma_(w[k], v0[k], a0);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) a1 = fm::fma_(w[k], v1[k], a1);
-        }
-        genc[(size_t)(2 * l) * n + q] = (T)a0;
-        genc[(size_t)(2 * l + 1) * n + q] = (T)a1;
+         q += gridDim.x * kLdsEncodeThreads)
+        lds_level_query<T>(c, lt, l, in, b, n, q, genc);
+}
+
+// The coarse levels of several queues (k_grid_encode_lds over each queue's
+// live queries; the counts are read on the device).
+template <class T>
+__global__ void __launch_bounds__(1024) k_grid_encode_lds_queues(const DevCfg c, const T* __restrict__ table,
+                                                                 const QueueSet qs) {
+    extern __shared__ uint32_t lt[];
+    const int l = blockIdx.y;
+    lds_stage_level(c, table, l, lt);
+    const int b = c.enc_begin[c.grid_enc];
+    for (uint32_t qi = 0; qi < qs.n_queues; ++qi) {
+        const uint32_t n = qs.stride[qi];
+        const uint32_t cnt = min(qs.counts[qi], n);
+        for (uint32_t q = blockIdx.x * kLdsEncodeThreads + threadIdx.x; q < cnt; q += gridDim.x * kLdsEncodeThreads)
+            lds_level_query<T>(c, lt, l, qs.io[qi].in, b, n, q, static_cast<T*>(qs.genc[qi]));
     }
 }
 
@@ -128,12 +158,12 @@ __global__ void __launch_bounds__(1024) k_grid_encode_lds(const DevCfg c, const 
 
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, const T* __restrict__ table,
-                                                            const QueueSet qs) {
+                                                            const QueueSet qs, int l0) {
     __shared__ uint32_t item;
     uint32_t first[kMaxQueues + 1];
     queue_tile_prefix(qs, 256u, first);
     const uint32_t per_level = first[kMaxQueues];
-    const uint32_t total = per_level * (uint32_t)c.n_levels;
+    const uint32_t total = per_level * (uint32_t)(c.n_levels - l0);  // levels l0 .. n_levels - 1
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(qs.counter, qs.batch);
         __syncthreads();
@@ -142,7 +172,7 @@ __global__ void __launch_bounds__(256) k_grid_encode_queues(const DevCfg c, cons
         if (w0 >= total) break;
         const uint32_t w1 = min(total, w0 + qs.batch);
         for (uint32_t w = w0; w < w1; ++w) {
-            const int l = (int)(w / per_level);
+            const int l = l0 + (int)(w / per_level);
             const uint32_t t = w % per_level;
             int qi = 0;
             uint32_t base = 0;
@@ -454,17 +484,32 @@ hipError_t launch_query_mfma_queues(const DevCfg& c, int precision, const void* 
     return hipErrorInvalidValue;
 }
 
+// capacity: the queues' total capacity; when it reaches the LDS batch size the
+// coarse levels go to k_grid_encode_lds_queues first.
 hipError_t launch_grid_encode_queues(const DevCfg& c, int precision, const void* table, const QueueSet& qs, int n_cu,
-                                     hipStream_t s) {
+                                     uint64_t capacity, hipStream_t s) {
     if (qs.n_queues == 0 || c.grid_enc < 0) return hipSuccess;
     if (c.F != 2 && c.F != 4) return hipErrorInvalidValue;
+    if (precision != PREC_BF16 && precision != PREC_FP16 && precision != PREC_BF16X3) return hipErrorInvalidValue;
+    const bool bf = precision == PREC_BF16;
+    const int n_lds = grid_encode_lds_levels(c, table, (uint32_t)std::min<uint64_t>(capacity, 0xffffffffu));
+    if (n_lds > 0) {
+        uint32_t lds = 0;
+        for (int l = 0; l < n_lds; ++l) lds = std::max(lds, (c.offsets[l + 1] - c.offsets[l]) * 4u);
+        const dim3 grid_l(256u, (uint32_t)n_lds);
+        if (bf)
+            hipLaunchKernelGGL((k_grid_encode_lds_queues<__bf16>), grid_l, dim3(kLdsEncodeThreads), lds, s, c,
+                               (const __bf16*)table, qs);
+        else
+            hipLaunchKernelGGL((k_grid_encode_lds_queues<_Float16>), grid_l, dim3(kLdsEncodeThreads), lds, s, c,
+                               (const _Float16*)table, qs);
+    }
     const dim3 grid((uint32_t)n_cu * 8u);
-    if (precision == PREC_BF16)
-        hipLaunchKernelGGL((k_grid_encode_queues<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, qs);
-    else if (precision == PREC_FP16 || precision == PREC_BF16X3)
-        hipLaunchKernelGGL((k_grid_encode_queues<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, qs);
+    if (bf)
+        hipLaunchKernelGGL((k_grid_encode_queues<__bf16>), grid, dim3(256), 0, s, c, (const __bf16*)table, qs, n_lds);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_grid_encode_queues<_Float16>), grid, dim3(256), 0, s, c, (const _Float16*)table, qs,
+                           n_lds);
     return hipGetLastError();
 }
 
